@@ -1,0 +1,89 @@
+// mi_common.hpp -- shared definitions of the gfx950 kernel library.
+//
+// Device layout of a symmetric matrix (the "packed lower tile" layout):
+// tile (m,n), m >= n, of an MT x MT tile grid lives at tile index
+// n*MT - n*(n-1)/2 + (m-n), each tile NB*NB elements, column-major, ld = NB.
+// It is the host layout of DPLASMA's lower symmetric block-cyclic descriptor
+// on one rank (tiles of mb*nb with lda = mb, src/include/dplasma/types_lapack.h:112),
+// so a lower descriptor stages in with one contiguous copy.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace mi {
+
+__host__ __device__ inline int64_t tile_index(int MT, int m, int n)
+{
+    return (int64_t)n * MT - ((int64_t)n * (n - 1)) / 2 + (m - n);
+}
+
+__host__ __device__ inline int tile_rows(int N, int NB, int MT, int m)
+{
+    return (m == MT - 1) ? N - m * NB : NB;
+}
+
+struct ProfRec {
+    const char *name;
+    hipEvent_t ev0, ev1;
+    double flops, bytes;
+};
+
+struct ProfAgg {
+    std::string name;
+    int launches = 0;
+    double ms = 0.0, flops = 0.0, bytes = 0.0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t s_main = nullptr;   // trailing updates, solves, copies
+    hipStream_t s_panel = nullptr;  // diagonal factorization + panel solve (look-ahead)
+    std::vector<hipEvent_t> ev_pool;   // cross-stream dependencies
+    size_t ev_next = 0;
+    int *d_info = nullptr;          // device-side first-failure info
+    double *d_scratch = nullptr;    // small reductions (norms)
+    size_t scratch_bytes = 0;
+    bool prof = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> prof_free;
+    std::vector<ProfAgg> agg;
+    hipEvent_t t0 = nullptr, t1 = nullptr;  // span of the last blocking call
+    double last_ms = 0.0;
+};
+
+void set_error(const char *fmt, ...);
+hipEvent_t next_event(Ctx *c);
+void prof_begin(Ctx *c, hipStream_t s, const char *name, double flops, double bytes, ProfRec *r);
+void prof_end(Ctx *c, hipStream_t s, ProfRec *r);
+
+}  // namespace mi
+
+// the opaque handle of the C ABI is the context itself
+struct mi_ctx_s : public mi::Ctx {};
+
+#define MI_CHECK(call)                                                            \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            mi::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,              \
+                          hipGetErrorString(e_));                                 \
+            return -1000 - (int)e_;                                               \
+        }                                                                         \
+    } while (0)
+
+// Launch with optional event bracketing for the profiler.
+#define MI_LAUNCH(ctx, stream, name, flops, bytes, ...)                           \
+    do {                                                                          \
+        mi::ProfRec r_{};                                                         \
+        if ((ctx)->prof) mi::prof_begin((ctx), (stream), (name), (flops), (bytes), &r_); \
+        __VA_ARGS__;                                                              \
+        hipError_t e_ = hipGetLastError();                                        \
+        if (e_ != hipSuccess) {                                                   \
+            mi::set_error("launch %s: %s", (name), hipGetErrorString(e_));        \
+            return -1000 - (int)e_;                                               \
+        }                                                                         \
+        if ((ctx)->prof) mi::prof_end((ctx), (stream), &r_);                      \
+    } while (0)

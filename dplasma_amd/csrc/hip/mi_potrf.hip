@@ -1,0 +1,566 @@
+// mi_potrf.hip -- tile Cholesky (lower) on gfx950, tiles resident in HBM.
+//
+// The four task classes of src/zpotrf_L.jdf become three kernels launched
+// per step k:
+//   k_potrf_diag   POTRF(k)        :91-208   one workgroup factors the
+//                                            diagonal tile with a 32-wide
+//                                            right-looking blocking, the
+//                                            current panel held in LDS; it also
+//                                            writes the inverses of the 32x32
+//                                            diagonal blocks for the panel solve
+//                                            and records info = k*NB + j.
+//   k_trsm_panel   TRSM(m,k), m>k  :214-343  C <- C L_kk^{-T}; one workgroup per
+//                                            64-row strip of a panel tile,
+//                                            blocked by 32 columns, diagonal
+//                                            blocks applied through their
+//                                            inverses.
+//   k_update_mfma  SYRK(k,m) and   :349-633  C(m,n) -= A(m,k) A(n,k)^T for every
+//                  GEMM(m,n,k)               trailing tile in a column range, on
+//                                            v_mfma_f64_16x16x4_f64 with
+//                                            double-buffered LDS staging; SYRK
+//                                            is the m == n case restricted to
+//                                            the lower triangle.
+// The host loop runs the panel (POTRF+TRSM of step k+1) on a second stream
+// as soon as column k+1 has received its step-k update, so it overlaps the
+// rest of the step-k trailing update (one-panel look-ahead).
+#include <cmath>
+#include "mi_common.hpp"
+#include "dplasma_mi.h"
+
+using namespace mi;
+
+namespace {
+
+constexpr int PJB = 32;            // panel width inside the diagonal factorization
+constexpr int POTRF_THREADS = 512;
+
+// ---------------------------------------------------------------------------
+// POTRF(k): factor the n x n diagonal tile A (ld) in place, lower.
+// LDS holds the current panel P (rows j0..n-1, PJB columns) column-major with
+// row stride RC.  Winv receives, for every 32-column block cb, the inverse of
+// the lower-triangular 32x32 diagonal block, column-major 32x32 at
+// Winv + cb*1024 (identity-padded past n).
+__global__ __launch_bounds__(POTRF_THREADS) void k_potrf_diag(double *__restrict__ A, int n, int ld,
+                                                             int RC, int info_base,
+                                                             int *__restrict__ d_info,
+                                                             double *__restrict__ Winv)
+{
+    extern __shared__ double smem[];
+    double *P = smem;                    // PJB * RC
+    double *Wl = smem + PJB * RC;        // 32 x 32 inverse being built
+    __shared__ int s_fail;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if (tid == 0) s_fail = 0;
+    __syncthreads();
+
+    for (int j0 = 0; j0 < n; j0 += PJB) {
+        const int jw = min(PJB, n - j0);
+        const int R = n - j0;
+        for (int idx = tid; idx < PJB * R; idx += nthr) {
+            int c = idx / R, r = idx - c * R;
+            P[c * RC + r] = (c < jw) ? A[(int64_t)(j0 + c) * ld + j0 + r] : 0.0;
+        }
+        __syncthreads();
+
+        // unblocked Cholesky of the jw x jw diagonal block by wave 0 (lane = row)
+        if (tid < 64) {
+            const int lane = tid;
+            for (int c = 0; c < jw; c++) {
+                double d = P[c * RC + c];
+                if (!(d > 0.0)) {
+                    if (lane == 0) s_fail = j0 + c + 1;
+                    break;
+                }
+                d = sqrt(d);
+                double lr = 0.0;
+                if (lane > c && lane < jw) {
+                    lr = P[c * RC + lane] / d;
+                    P[c * RC + lane] = lr;
+                }
+                if (lane == c) P[c * RC + c] = d;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane > c && lane < jw)
+                    for (int c2 = c + 1; c2 <= lane; c2++) P[c2 * RC + lane] -= lr * P[c * RC + c2];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        __syncthreads();
+        if (s_fail) break;
+
+        // inverse of the diagonal block: lane j < 32 solves L w = e_j
+        if (tid < 32) {
+            const int j = tid;
+            double w[PJB];
+#pragma unroll
+            for (int i = 0; i < PJB; i++) w[i] = 0.0;
+#pragma unroll
+            for (int i = 0; i < PJB; i++) {
+                if (i < jw) {
+                    double s = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+                    for (int p = 0; p < PJB; p++)
+                        if (p < i) s -= P[p * RC + i] * w[p];
+                    w[i] = (i >= j) ? s / P[i * RC + i] : 0.0;
+                } else {
+                    w[i] = (i == j) ? 1.0 : 0.0;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < PJB; i++) Wl[j * 32 + i] = w[i];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < 1024; idx += nthr) Winv[(j0 / PJB) * 1024 + idx] = Wl[idx];
+
+        // panel rows below the diagonal block: x = b L11^{-T}, i.e. x = b W^T
+        for (int r = jw + tid; r < R; r += nthr) {
+            double b[PJB];
+#pragma unroll
+            for (int c = 0; c < PJB; c++) b[c] = P[c * RC + r];
+#pragma unroll
+            for (int c = 0; c < PJB; c++) {
+                double s = 0.0;
+#pragma unroll
+                for (int p = 0; p < PJB; p++)
+                    if (p <= c) s += b[p] * Wl[p * 32 + c];
+                P[c * RC + r] = (c < jw) ? s : 0.0;
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < jw * R; idx += nthr) {
+            int c = idx / R, r = idx - c * R;
+            if (r >= c) A[(int64_t)(j0 + c) * ld + j0 + r] = P[c * RC + r];
+        }
+
+        // trailing update inside the tile: A22 -= P2 P2^T (lower), 4x4 per thread
+        const int M2 = R - jw;
+        if (M2 > 0) {
+            const int T4 = (M2 + 3) >> 2;
+            const int ntile = T4 * (T4 + 1) / 2;
+            for (int t = tid; t < ntile; t += nthr) {
+                int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+                while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
+                while (ti * (ti + 1) / 2 > t) ti--;
+                const int tj = t - ti * (ti + 1) / 2;
+                const int r0 = jw + 4 * ti, c0 = jw + 4 * tj;
+                double acc[4][4];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+                for (int p = 0; p < jw; p++) {
+                    const double *pc = P + p * RC;
+                    double a[4], bb[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) { a[i] = pc[r0 + i]; bb[i] = pc[c0 + i]; }
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * bb[j];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int cc = c0 + j;
+                    if (cc >= R) continue;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int rr = r0 + i;
+                        if (rr < R && rr >= cc) A[(int64_t)(j0 + cc) * ld + j0 + rr] -= acc[i][j];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && s_fail) atomicCAS(d_info, 0, info_base + s_fail);
+}
+
+// ---------------------------------------------------------------------------
+// TRSM(m,k) for m = k+1 .. MT-1: C <- C L^{-T}.  Grid: (MT-k-1) tiles x
+// ceil(rows/64) strips.  L = diagonal tile (ld NB), Winv = its 32x32
+// diagonal-block inverses.  256 threads: row = t % 64, column group g = t / 64
+// owns 8 of the 32 columns of the current block.
+constexpr int TRSM_ROWS = 64;
+__global__ __launch_bounds__(256) void k_trsm_panel(double *__restrict__ T, int N, int NB, int MT, int k,
+                                                   const double *__restrict__ Winv)
+{
+    __shared__ double Xs[32][TRSM_ROWS + 1];   // X chunk, [p][row]
+    __shared__ double Ls[32][33];              // L chunk, [p][col]
+    __shared__ double Ys[TRSM_ROWS][33];       // block right-hand side [row][col]
+    __shared__ double Ws[32][33];              // inverse of the diagonal block [col][row]
+    const int tid = threadIdx.x;
+    const int row = tid & 63, g = tid >> 6;
+    const int strips = (NB + TRSM_ROWS - 1) / TRSM_ROWS;
+    const int m = k + 1 + blockIdx.x / strips;
+    const int strip = blockIdx.x % strips;
+    const int rows_m = tile_rows(N, NB, MT, m);
+    const int r0 = strip * TRSM_ROWS;
+    if (r0 >= rows_m) return;
+    const int rv = min(TRSM_ROWS, rows_m - r0);
+    double *C = T + tile_index(MT, m, k) * (int64_t)NB * NB + r0;     // strip origin
+    const double *L = T + tile_index(MT, k, k) * (int64_t)NB * NB;
+
+    for (int c0 = 0; c0 < NB; c0 += 32) {
+        const int cw = min(32, NB - c0);
+        double acc[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = g * 8 + j;
+            acc[j] = (row < rv && c < cw) ? C[(int64_t)(c0 + c) * NB + row] : 0.0;
+        }
+        for (int p0 = 0; p0 < c0; p0 += 32) {
+            __syncthreads();
+            for (int idx = tid; idx < 32 * TRSM_ROWS; idx += 256) {
+                int p = idx / TRSM_ROWS, r = idx % TRSM_ROWS;
+                Xs[p][r] = (r < rv) ? C[(int64_t)(p0 + p) * NB + r] : 0.0;
+            }
+            for (int idx = tid; idx < 32 * 32; idx += 256) {
+                int p = idx / 32, c = idx % 32;
+                Ls[p][c] = (c < cw) ? L[(int64_t)(p0 + p) * NB + c0 + c] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int p = 0; p < 32; p++) {
+                const double x = Xs[p][row];
+#pragma unroll
+                for (int j = 0; j < 8; j++) acc[j] -= x * Ls[p][g * 8 + j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 8; j++) Ys[row][g * 8 + j] = acc[j];
+        for (int idx = tid; idx < 1024; idx += 256) {
+            int col = idx / 32, r = idx % 32;
+            Ws[col][r] = Winv[(c0 / 32) * 1024 + idx];   // W(r, col)
+        }
+        __syncthreads();
+        // X(row, c) = sum_{p<=c} Y(row, p) W(c, p)
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = g * 8 + j;
+            double s = 0.0;
+            for (int p = 0; p <= c; p++) s += Ys[row][p] * Ws[p][c];
+            if (row < rv && c < cw) C[(int64_t)(c0 + c) * NB + row] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Trailing update on MFMA: C(m,n) -= A(m,k) A(n,k)^T for tile columns n in
+// [n_begin, n_end), rows m in [n, MT).  One workgroup = one 128x128 block of
+// one tile; four waves, each a 64x64 quadrant as 4x4 MFMA 16x16 tiles.
+constexpr int BM = 128, BK = 16;
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_update_mfma(double *__restrict__ T, int N, int NB, int MT, int k,
+                                                    int n_begin, int nsub)
+{
+    __shared__ double As[2][BK][BM];
+    __shared__ double Bs[2][BK][BM];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    // XCD-aware remap: consecutive workgroup ids land on different XCDs
+    // (round-robin), so give each XCD a contiguous range of blocks; the
+    // blocks of one tile then share their A/B panels in that XCD's L2.
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int G8 = (G / 8) * 8;
+    const int L = (bid < G8) ? (bid % 8) * (G8 / 8) + bid / 8 : bid;
+
+    const int per_tile = nsub * nsub;
+    int t = L / per_tile;
+    const int s = L - t * per_tile;
+    int n = n_begin;
+    while (t >= MT - n) { t -= MT - n; n++; }
+    const int m = n + t;
+    const int bm = s / nsub, bn = s - bm * nsub;
+    if (m == n && bm < bn) return;                 // strictly upper blocks of a diagonal tile
+
+    const int rows_m = tile_rows(N, NB, MT, m);
+    const int cols_n = tile_rows(N, NB, MT, n);
+    const int r_base = bm * BM, c_base = bn * BM;
+    if (r_base >= rows_m || c_base >= cols_n) return;
+
+    const double *Ap = T + tile_index(MT, m, k) * (int64_t)NB * NB;
+    const double *Bp = T + tile_index(MT, n, k) * (int64_t)NB * NB;
+    double *Cp = T + tile_index(MT, m, n) * (int64_t)NB * NB;
+
+    dx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
+
+    // global -> register staging: element e = tid + 256*i, kk = e / 128, r = e % 128
+    double ra[8], rb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int e = tid + 256 * i;
+            const int kk = e >> 7, r = e & 127;
+            const int kg = k0 + kk;
+            const int ra_row = r_base + r, rb_row = c_base + r;
+            ra[i] = (kg < NB && ra_row < rows_m) ? Ap[(int64_t)kg * NB + ra_row] : 0.0;
+            rb[i] = (kg < NB && rb_row < cols_n) ? Bp[(int64_t)kg * NB + rb_row] : 0.0;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int e = tid + 256 * i;
+            As[buf][e >> 7][e & 127] = ra[i];
+            Bs[buf][e >> 7][e & 127] = rb[i];
+        }
+    };
+
+    const int nstage = (NB + BK - 1) / BK;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int fr = lane & 15, fk = lane >> 4;
+    for (int st = 0; st < nstage; st++) {
+        const int buf = st & 1;
+        if (st + 1 < nstage) gload((st + 1) * BK);
+#pragma unroll
+        for (int kq = 0; kq < BK / 4; kq++) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = As[buf][kq * 4 + fk][wm * 64 + i * 16 + fr];
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = Bs[buf][kq * 4 + fk][wn * 64 + j * 16 + fr];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (st + 1 < nstage) lstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue: lane holds rows 4*(lane/16)+q, column lane%16 of each 16x16 tile
+    const bool diag = (m == n) && (bm == bn);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int col = c_base + wn * 64 + j * 16 + fr;
+            if (col >= cols_n) continue;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = r_base + wm * 64 + i * 16 + 4 * fk + q;
+                if (rr < rows_m && (!diag || rr >= col))
+                    Cp[(int64_t)col * NB + rr] -= acc[i][j][q];
+            }
+        }
+}
+
+__global__ void k_fill_int(int *p, int v, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+struct Plan {
+    int N, NB, MT, RC;
+    size_t potrf_lds;
+};
+
+int potrf_lds_bytes(int NB, int *RC)
+{
+    *RC = ((NB + 3) & ~3) + 4;
+    return (PJB * (*RC) + 1024) * (int)sizeof(double);
+}
+
+double update_flops(int N, int NB, int MT, int n_begin, int n_end)
+{
+    double f = 0.0;
+    for (int n = n_begin; n < n_end; n++)
+        for (int m = n; m < MT; m++) {
+            double tm = tile_rows(N, NB, MT, m);
+            if (m == n) f += (double)NB * tm * (tm + 1.0);
+            else f += 2.0 * tm * NB * NB;
+        }
+    return f;
+}
+
+double update_bytes(int N, int NB, int MT, int n_begin, int n_end)
+{
+    // C read + written once, each panel tile of column k read once
+    double b = 0.0;
+    for (int n = n_begin; n < n_end; n++)
+        for (int m = n; m < MT; m++) b += 2.0 * tile_rows(N, NB, MT, m) * NB * 8.0;
+    return b + (double)(MT - n_begin) * NB * NB * 8.0;
+}
+
+int launch_potrf(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, double *Winv)
+{
+    int RC;
+    int lds = potrf_lds_bytes(NB, &RC);
+    const int n = tile_rows(N, NB, MT, k);
+    double *A = dT + tile_index(MT, k, k) * (int64_t)NB * NB;
+    const double fl = (double)n * n * n / 3.0;
+    MI_LAUNCH(c, s, "dpotrf_potrf_diag", fl, 2.0 * n * n * 8.0,
+              k_potrf_diag<<<1, POTRF_THREADS, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
+    return 0;
+}
+
+int launch_trsm(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, const double *Winv)
+{
+    const int ntile = MT - k - 1;
+    if (ntile <= 0) return 0;
+    const int strips = (NB + TRSM_ROWS - 1) / TRSM_ROWS;
+    double fl = 0.0;
+    for (int m = k + 1; m < MT; m++) fl += (double)tile_rows(N, NB, MT, m) * NB * NB;
+    MI_LAUNCH(c, s, "dpotrf_trsm_panel", fl, fl / NB * 16.0,
+              k_trsm_panel<<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+    return 0;
+}
+
+int launch_update(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, int n_begin,
+                  int n_end)
+{
+    if (n_begin >= n_end) return 0;
+    const int nsub = (NB + BM - 1) / BM;
+    int64_t ntiles = 0;
+    for (int n = n_begin; n < n_end; n++) ntiles += MT - n;
+    const int64_t grid = ntiles * nsub * nsub;
+    if (grid <= 0) return 0;
+    if (grid > 0x7fffffff) { set_error("update grid too large"); return -3; }
+    MI_LAUNCH(c, s, "dpotrf_update_mfma", update_flops(N, NB, MT, n_begin, n_end),
+              update_bytes(N, NB, MT, n_begin, n_end),
+              k_update_mfma<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, nsub));
+    return 0;
+}
+
+
+int prepare_potrf(int NB)
+{
+    int RC;
+    int lds = potrf_lds_bytes(NB, &RC);
+    int dev = 0;
+    MI_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MI_CHECK(hipGetDeviceProperties(&prop, dev));
+    size_t limit = prop.sharedMemPerBlock;
+    if ((size_t)lds > limit) {
+        set_error("tile size NB=%d needs %d B of LDS for the diagonal factorization (limit %zu)",
+                  NB, lds, limit);
+        return -4;
+    }
+    if (lds > 65536)
+        MI_CHECK(hipFuncSetAttribute((const void *)k_potrf_diag,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    return 0;
+}
+
+struct Workspace {
+    double *winv = nullptr;
+    size_t bytes = 0;
+};
+Workspace g_ws;
+
+int ensure_ws(size_t bytes)
+{
+    if (g_ws.bytes >= bytes) return 0;
+    if (g_ws.winv) (void)hipFree(g_ws.winv);
+    g_ws.winv = nullptr;
+    g_ws.bytes = 0;
+    MI_CHECK(hipMalloc(&g_ws.winv, bytes));
+    g_ws.bytes = bytes;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mi_dpotrf_step_potrf(mi_ctx_t *c, int N, int NB, double *dT, int k, int *info)
+{
+    if (N <= 0) { *info = 0; return 0; }
+    const int MT = (N + NB - 1) / NB;
+    int rc = prepare_potrf(NB);
+    if (rc) return rc;
+    const int nblk = (NB + 31) / 32;
+    if ((rc = ensure_ws((size_t)2 * nblk * 1024 * sizeof(double)))) return rc;
+    k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
+    if ((rc = launch_potrf(c, c->s_main, dT, N, NB, MT, k, g_ws.winv))) return rc;
+    MI_CHECK(hipMemcpyAsync(info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
+    return mi_synchronize(c);
+}
+
+int mi_dpotrf_step_trsm(mi_ctx_t *c, int N, int NB, double *dT, int k)
+{
+    // the diagonal-block inverses come from the POTRF launch of the same k
+    const int MT = (N + NB - 1) / NB;
+    int rc = launch_trsm(c, c->s_main, dT, N, NB, MT, k, g_ws.winv);
+    if (rc) return rc;
+    return mi_synchronize(c);
+}
+
+int mi_dpotrf_step_update(mi_ctx_t *c, int N, int NB, double *dT, int k, int n_begin, int n_end)
+{
+    const int MT = (N + NB - 1) / NB;
+    if (k < 0 || n_begin <= k || n_end > MT) { set_error("bad update range"); return -2; }
+    int rc = launch_update(c, c->s_main, dT, N, NB, MT, k, n_begin, n_end);
+    if (rc) return rc;
+    return mi_synchronize(c);
+}
+
+int mi_dpotrf_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
+{
+    *info = 0;
+    if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
+    if (N == 0) return 0;
+    const int MT = (N + NB - 1) / NB;
+    int rc = prepare_potrf(NB);
+    if (rc) return rc;
+    const int nblk = (NB + 31) / 32;
+    const size_t wsz = (size_t)nblk * 1024;        // doubles per step
+    if ((rc = ensure_ws(2 * wsz * sizeof(double)))) return rc;
+    c->ev_next = 0;
+
+    MI_CHECK(hipEventRecord(c->t0, c->s_main));
+    k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
+    hipEvent_t start = next_event(c);
+    MI_CHECK(hipEventRecord(start, c->s_main));
+    MI_CHECK(hipStreamWaitEvent(c->s_panel, start, 0));
+
+    // panel 0
+    if ((rc = launch_potrf(c, c->s_panel, dT, N, NB, MT, 0, g_ws.winv))) return rc;
+    if ((rc = launch_trsm(c, c->s_panel, dT, N, NB, MT, 0, g_ws.winv))) return rc;
+    hipEvent_t pk = next_event(c);
+    MI_CHECK(hipEventRecord(pk, c->s_panel));
+
+    for (int k = 0; k < MT - 1; k++) {
+        MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
+        // column k+1 first: it gates the next panel
+        if ((rc = launch_update(c, c->s_main, dT, N, NB, MT, k, k + 1, k + 2))) return rc;
+        hipEvent_t ck = next_event(c);
+        MI_CHECK(hipEventRecord(ck, c->s_main));
+        MI_CHECK(hipStreamWaitEvent(c->s_panel, ck, 0));
+        double *w = g_ws.winv + ((k + 1) & 1) * wsz;
+        if ((rc = launch_potrf(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
+        if ((rc = launch_trsm(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
+        pk = next_event(c);
+        MI_CHECK(hipEventRecord(pk, c->s_panel));
+        // the rest of the trailing matrix overlaps the panel above
+        if ((rc = launch_update(c, c->s_main, dT, N, NB, MT, k, k + 2, MT))) return rc;
+    }
+    MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
+    MI_CHECK(hipEventRecord(c->t1, c->s_main));
+    int h_info = 0;
+    MI_CHECK(hipMemcpyAsync(&h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
+    if ((rc = mi_synchronize(c))) return rc;
+    float ms = 0.f;
+    MI_CHECK(hipEventElapsedTime(&ms, c->t0, c->t1));
+    c->last_ms = ms;
+    *info = h_info;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+// mi_runtime.hip -- context, memory, copies and the event profiler of the
+// gfx950 kernel library (C ABI in include/dplasma_mi.h).
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include "mi_common.hpp"
+#include "dplasma_mi.h"
+
+namespace mi {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+hipEvent_t next_event(Ctx *c)
+{
+    if (c->ev_next >= c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_next++];
+}
+
+static hipEvent_t prof_event(Ctx *c)
+{
+    if (!c->prof_free.empty()) {
+        hipEvent_t e = c->prof_free.back();
+        c->prof_free.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void prof_begin(Ctx *c, hipStream_t s, const char *name, double flops, double bytes, ProfRec *r)
+{
+    r->name = name;
+    r->flops = flops;
+    r->bytes = bytes;
+    r->ev0 = prof_event(c);
+    r->ev1 = prof_event(c);
+    (void)hipEventRecord(r->ev0, s);
+}
+
+void prof_end(Ctx *c, hipStream_t s, ProfRec *r)
+{
+    (void)hipEventRecord(r->ev1, s);
+    c->recs.push_back(*r);
+}
+
+// Resolve recorded launches into per-name totals (after a synchronize).
+static void prof_collect(Ctx *c)
+{
+    for (auto &r : c->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.ev0, r.ev1) != hipSuccess) ms = 0.f;
+        ProfAgg *a = nullptr;
+        for (auto &x : c->agg)
+            if (x.name == r.name) { a = &x; break; }
+        if (!a) {
+            c->agg.push_back(ProfAgg());
+            a = &c->agg.back();
+            a->name = r.name;
+        }
+        a->launches++;
+        a->ms += ms;
+        a->flops += r.flops;
+        a->bytes += r.bytes;
+        c->prof_free.push_back(r.ev0);
+        c->prof_free.push_back(r.ev1);
+    }
+    c->recs.clear();
+}
+
+}  // namespace mi
+
+using namespace mi;
+
+extern "C" {
+
+const char *mi_last_error(void) { return mi::g_err; }
+
+int mi_device_count(int *count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { *count = 0; set_error("hipGetDeviceCount: %s", hipGetErrorString(e)); return -1; }
+    *count = n;
+    return 0;
+}
+
+int mi_ctx_create(int device, mi_ctx_t **out)
+{
+    *out = nullptr;
+    int n = 0;
+    if (mi_device_count(&n) != 0 || n <= 0) { set_error("no HIP device visible"); return -1; }
+    if (device < 0 || device >= n) { set_error("device %d out of range (%d devices)", device, n); return -2; }
+    MI_CHECK(hipSetDevice(device));
+    mi_ctx_t *c = new mi_ctx_t();
+    c->device = device;
+    // default priority on both streams: the look-ahead relies on ordinary
+    // queue arbitration, not on stream priorities
+    MI_CHECK(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
+    MI_CHECK(hipStreamCreateWithFlags(&c->s_panel, hipStreamNonBlocking));
+    MI_CHECK(hipMalloc(&c->d_info, 64 * sizeof(int)));
+    c->scratch_bytes = 1 << 20;
+    MI_CHECK(hipMalloc(&c->d_scratch, c->scratch_bytes));
+    MI_CHECK(hipEventCreate(&c->t0));
+    MI_CHECK(hipEventCreate(&c->t1));
+    *out = c;
+    return 0;
+}
+
+void mi_ctx_destroy(mi_ctx_t *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->s_main);
+    (void)hipStreamSynchronize(c->s_panel);
+    mi::prof_collect(c);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto e : c->prof_free) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(c->t0);
+    (void)hipEventDestroy(c->t1);
+    (void)hipFree(c->d_info);
+    (void)hipFree(c->d_scratch);
+    (void)hipStreamDestroy(c->s_main);
+    (void)hipStreamDestroy(c->s_panel);
+    delete c;
+}
+
+int mi_synchronize(mi_ctx_t *c)
+{
+    MI_CHECK(hipStreamSynchronize(c->s_panel));
+    MI_CHECK(hipStreamSynchronize(c->s_main));
+    c->ev_next = 0;
+    if (c->prof) mi::prof_collect(c);
+    return 0;
+}
+
+int mi_malloc(mi_ctx_t *c, void **p, size_t bytes)
+{
+    (void)c;
+    *p = nullptr;
+    if (bytes == 0) return 0;
+    MI_CHECK(hipMalloc(p, bytes));
+    return 0;
+}
+
+int mi_free(mi_ctx_t *c, void *p)
+{
+    (void)c;
+    if (p) MI_CHECK(hipFree(p));
+    return 0;
+}
+
+int mi_host_alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return 0;
+    MI_CHECK(hipHostMalloc(p, bytes, hipHostMallocDefault));
+    return 0;
+}
+
+int mi_host_free(void *p)
+{
+    if (p) MI_CHECK(hipHostFree(p));
+    return 0;
+}
+
+int mi_memcpy_h2d(mi_ctx_t *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    MI_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->s_main));
+    MI_CHECK(hipStreamSynchronize(c->s_main));
+    return 0;
+}
+
+int mi_memcpy_d2h(mi_ctx_t *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    MI_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->s_main));
+    MI_CHECK(hipStreamSynchronize(c->s_main));
+    return 0;
+}
+
+int mi_memcpy_d2d(mi_ctx_t *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    MI_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->s_main));
+    MI_CHECK(hipStreamSynchronize(c->s_main));
+    return 0;
+}
+
+int mi_memcpy2d_h2d(mi_ctx_t *c, void *dst, size_t dpitch, const void *src, size_t spitch,
+                    size_t width, size_t height)
+{
+    if (!width || !height) return 0;
+    MI_CHECK(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyHostToDevice,
+                              c->s_main));
+    MI_CHECK(hipStreamSynchronize(c->s_main));
+    return 0;
+}
+
+int mi_memcpy2d_d2h(mi_ctx_t *c, void *dst, size_t dpitch, const void *src, size_t spitch,
+                    size_t width, size_t height)
+{
+    if (!width || !height) return 0;
+    MI_CHECK(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost,
+                              c->s_main));
+    MI_CHECK(hipStreamSynchronize(c->s_main));
+    return 0;
+}
+
+int mi_memset(mi_ctx_t *c, void *p, int v, size_t bytes)
+{
+    if (!bytes) return 0;
+    MI_CHECK(hipMemsetAsync(p, v, bytes, c->s_main));
+    MI_CHECK(hipStreamSynchronize(c->s_main));
+    return 0;
+}
+
+int mi_prof_enable(mi_ctx_t *c, int on)
+{
+    c->prof = (on != 0);
+    return 0;
+}
+
+int mi_prof_reset(mi_ctx_t *c)
+{
+    mi_synchronize(c);
+    c->agg.clear();
+    return 0;
+}
+
+int mi_prof_get(mi_ctx_t *c, const char *kernel, int *launches, double *ms, double *flops,
+                double *bytes)
+{
+    mi::prof_collect(c);
+    for (auto &a : c->agg)
+        if (a.name == kernel) {
+            if (launches) *launches = a.launches;
+            if (ms) *ms = a.ms;
+            if (flops) *flops = a.flops;
+            if (bytes) *bytes = a.bytes;
+            return 0;
+        }
+    if (launches) *launches = 0;
+    if (ms) *ms = 0;
+    if (flops) *flops = 0;
+    if (bytes) *bytes = 0;
+    return -1;
+}
+
+int mi_prof_count(mi_ctx_t *c)
+{
+    mi::prof_collect(c);
+    return (int)c->agg.size();
+}
+
+const char *mi_prof_name(mi_ctx_t *c, int i)
+{
+    if (i < 0 || i >= (int)c->agg.size()) return nullptr;
+    return c->agg[i].name.c_str();
+}
+
+int mi_last_span_ms(mi_ctx_t *c, double *ms)
+{
+    *ms = c->last_ms;
+    return 0;
+}
+
+}  // extern "C"

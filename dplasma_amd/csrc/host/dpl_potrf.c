@@ -1,0 +1,246 @@
+/*
+ * dpl_potrf.c -- dplasma_dpotrf / dpotrs / dposv / dtrsm on the GPU.
+ *
+ * Argument checks and return conventions follow the reference wrappers:
+ *   dplasma_zpotrf      src/zpotrf_wrapper.c:345-371 (New :193-258)
+ *   dplasma_zpotrs      src/zpotrs_wrapper.c:56-97
+ *   dplasma_zposv       src/zposv_wrapper.c:63-113
+ *   dplasma_ztrsm       src/ztrsm_wrapper.c:262-309 (New :93-162)
+ * The operand images live in HBM for the whole call (or across calls when
+ * advised); the work itself is the kernel library's mi_dpotrf_tiles /
+ * mi_dtrsm_LL / mi_dpotrs_tiles.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include "dpl_internal.h"
+
+/* ----------------------------------------------------------- executors */
+int dpl_run_potrf(parsec_context_t *ctx, int uplo, parsec_tiled_matrix_t *A, int *info)
+{
+    const char *f = "dplasma_dpotrf";
+    dpl_sym_dev_t s;
+    memset(&ctx->last, 0, sizeof ctx->last);
+    int rc = dpl_sym_acquire(ctx, A, uplo, 1, &s, &ctx->last.stage_in_ms);
+    if (rc) return rc;
+    int iinfo = 0;
+    rc = mi_dpotrf_tiles(ctx->mi, s.N, s.NB, s.d, &iinfo);
+    if (rc) {
+        dpl_sym_release(ctx, A, &s, 0, NULL);
+        return dpl_mi_fail(f, rc);
+    }
+    mi_last_span_ms(ctx->mi, &ctx->last.total_ms);
+    if (info && *info == 0) *info = iinfo;
+    return dpl_sym_release(ctx, A, &s, 1, &ctx->last.stage_out_ms);
+}
+
+/* map (uplo, trans) of the stored triangle onto the lower image:
+ * Upper holds U = L^T, so U X = B is L^T X = B and U^T X = B is L X = B */
+static int lower_trans(int uplo, int trans)
+{
+    int t = (trans == dplasmaNoTrans) ? dplasmaNoTrans : dplasmaTrans;
+    if (uplo == dplasmaUpper) t = (t == dplasmaNoTrans) ? dplasmaTrans : dplasmaNoTrans;
+    return t;
+}
+
+int dpl_run_trsm(parsec_context_t *ctx, int side, int uplo, int trans, int diag, double alpha,
+                 const parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B)
+{
+    const char *f = "dplasma_dtrsm";
+    if (side != dplasmaLeft) {
+        dpl_error(f, "side = Right is not provided by this library");
+        return -101;
+    }
+    dpl_sym_dev_t s;
+    dpl_ge_dev_t g;
+    memset(&ctx->last, 0, sizeof ctx->last);
+    int rc = dpl_sym_acquire(ctx, A, uplo, 1, &s, &ctx->last.stage_in_ms);
+    if (rc) return rc;
+    if (B->mb != A->mb && B->mt > 1) {
+        dpl_sym_release(ctx, A, &s, 0, NULL);
+        dpl_error(f, "B must use the row tile size of A");
+        return -6;
+    }
+    if ((rc = dpl_ge_acquire(ctx, B, 1, &g, &ctx->last.stage_in_ms))) {
+        dpl_sym_release(ctx, A, &s, 0, NULL);
+        return rc;
+    }
+    rc = mi_dtrsm_LL(ctx->mi, lower_trans(uplo, trans), diag, alpha, s.N, s.NB, s.d, g.N, g.d, g.ld);
+    if (rc) rc = dpl_mi_fail(f, rc);
+    else mi_last_span_ms(ctx->mi, &ctx->last.total_ms);
+    int r2 = dpl_ge_release(ctx, B, &g, rc == 0, &ctx->last.stage_out_ms);
+    dpl_sym_release(ctx, A, &s, 0, NULL);
+    return rc ? rc : r2;
+}
+
+/* --------------------------------------------------------------- potrf */
+parsec_taskpool_t *dplasma_dpotrf_New(dplasma_enum_t uplo, parsec_tiled_matrix_t *A, int *info)
+{
+    if (uplo != dplasmaUpper && uplo != dplasmaLower) {
+        dpl_error("dplasma_dpotrf_New", "illegal value of uplo");
+        return NULL;
+    }
+    *info = 0;
+    parsec_taskpool_t *tp = (parsec_taskpool_t *)calloc(1, sizeof(parsec_taskpool_t));
+    tp->kind = DPL_TP_POTRF;
+    tp->uplo = uplo;
+    tp->A = A;
+    tp->info = info;
+    return tp;
+}
+
+void dplasma_dpotrf_Destruct(parsec_taskpool_t *tp) { free(tp); }
+
+void dplasma_dpotrf_setrecursive(parsec_taskpool_t *tp, int hmb)
+{
+    /* the diagonal tile is already split into 32-column panels inside one
+     * workgroup; the recursive tile size only has to be recorded */
+    if (tp && hmb > 0) tp->hmb = hmb;
+}
+
+int dplasma_dpotrf(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A)
+{
+    int info = 0;
+    parsec_taskpool_t *tp = dplasma_dpotrf_New(uplo, A, &info);
+    if (tp) {
+        parsec_context_add_taskpool(parsec, tp);
+        int rc = parsec_context_wait(parsec);
+        dplasma_dpotrf_Destruct(tp);
+        if (rc) return rc;
+    }
+    return info;
+}
+
+int dplasma_dpotrf_rec(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A, int hmb)
+{
+    int info = 0;
+    parsec_taskpool_t *tp = dplasma_dpotrf_New(uplo, A, &info);
+    if (tp) {
+        dplasma_dpotrf_setrecursive(tp, hmb);
+        parsec_context_add_taskpool(parsec, tp);
+        int rc = parsec_context_wait(parsec);
+        dplasma_dpotrf_Destruct(tp);
+        if (rc) return rc;
+    }
+    return info;
+}
+
+/* ---------------------------------------------------------------- trsm */
+parsec_taskpool_t *dplasma_dtrsm_New(dplasma_enum_t side, dplasma_enum_t uplo, dplasma_enum_t trans,
+                                     dplasma_enum_t diag, double alpha, const parsec_tiled_matrix_t *A,
+                                     parsec_tiled_matrix_t *B)
+{
+    if (side != dplasmaLeft) {
+        dpl_error("dplasma_dtrsm_New", "side = Right is not provided by this library");
+        return NULL;
+    }
+    parsec_taskpool_t *tp = (parsec_taskpool_t *)calloc(1, sizeof(parsec_taskpool_t));
+    tp->kind = DPL_TP_TRSM;
+    tp->side = side;
+    tp->uplo = uplo;
+    tp->trans = trans;
+    tp->diag = diag;
+    tp->alpha = alpha;
+    tp->Ac = A;
+    tp->B = B;
+    return tp;
+}
+
+void dplasma_dtrsm_Destruct(parsec_taskpool_t *tp) { free(tp); }
+
+int dplasma_dtrsm(parsec_context_t *parsec, dplasma_enum_t side, dplasma_enum_t uplo, dplasma_enum_t trans,
+                  dplasma_enum_t diag, double alpha, const parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B)
+{
+    if (side != dplasmaLeft && side != dplasmaRight) {
+        dpl_error("dplasma_dtrsm", "illegal value of side");
+        return -1;
+    }
+    if (uplo != dplasmaUpper && uplo != dplasmaLower) {
+        dpl_error("dplasma_dtrsm", "illegal value of uplo");
+        return -2;
+    }
+    if (trans != dplasmaConjTrans && trans != dplasmaNoTrans && trans != dplasmaTrans) {
+        dpl_error("dplasma_dtrsm", "illegal value of trans");
+        return -3;
+    }
+    if (diag != dplasmaUnit && diag != dplasmaNonUnit) {
+        dpl_error("dplasma_dtrsm", "illegal value of diag");
+        return -4;
+    }
+    if ((A->m != A->n) || ((side == dplasmaLeft) && (A->n != B->m)) ||
+        ((side == dplasmaRight) && (A->n != B->n))) {
+        dpl_error("dplasma_dtrsm", "illegal matrix A");
+        return -6;
+    }
+    parsec_taskpool_t *tp = dplasma_dtrsm_New(side, uplo, trans, diag, alpha, A, B);
+    if (!tp) return -101;
+    parsec_context_add_taskpool(parsec, tp);
+    int rc = parsec_context_wait(parsec);
+    dplasma_dtrsm_Destruct(tp);
+    return rc;
+}
+
+/* ------------------------------------------------------------- potrs/posv */
+int dplasma_dpotrs(parsec_context_t *parsec, dplasma_enum_t uplo, const parsec_tiled_matrix_t *A,
+                   parsec_tiled_matrix_t *B)
+{
+    if (uplo != dplasmaUpper && uplo != dplasmaLower) {
+        dpl_error("dplasma_dpotrs", "illegal value of uplo");
+        return -1;
+    }
+    /* L L^T X = B as two triangular solves on one staged image of A
+     * (zpotrs_wrapper.c:88-94: Upper = ConjTrans then NoTrans, Lower =
+     * NoTrans then ConjTrans; both are L then L^T on the lower image) */
+    dpl_sym_dev_t s;
+    dpl_ge_dev_t g;
+    memset(&parsec->last, 0, sizeof parsec->last);
+    int rc = dpl_sym_acquire(parsec, A, uplo, 1, &s, &parsec->last.stage_in_ms);
+    if (rc) return rc;
+    if ((rc = dpl_ge_acquire(parsec, B, 1, &g, &parsec->last.stage_in_ms))) {
+        dpl_sym_release(parsec, A, &s, 0, NULL);
+        return rc;
+    }
+    rc = mi_dpotrs_tiles(parsec->mi, s.N, s.NB, s.d, g.N, g.d, g.ld);
+    if (rc) rc = dpl_mi_fail("dplasma_dpotrs", rc);
+    else mi_last_span_ms(parsec->mi, &parsec->last.total_ms);
+    int r2 = dpl_ge_release(parsec, B, &g, rc == 0, &parsec->last.stage_out_ms);
+    dpl_sym_release(parsec, A, &s, 0, NULL);
+    return rc ? rc : r2;
+}
+
+int dplasma_dposv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A,
+                  parsec_tiled_matrix_t *B)
+{
+    if (uplo != dplasmaUpper && uplo != dplasmaLower) {
+        dpl_error("dplasma_dposv", "illegal value of uplo");
+        return -1;
+    }
+    dpl_sym_dev_t s;
+    dpl_ge_dev_t g;
+    memset(&parsec->last, 0, sizeof parsec->last);
+    int rc = dpl_sym_acquire(parsec, A, uplo, 1, &s, &parsec->last.stage_in_ms);
+    if (rc) return rc;
+    int info = 0;
+    double ms_f = 0.0, ms_s = 0.0;
+    rc = mi_dpotrf_tiles(parsec->mi, s.N, s.NB, s.d, &info);
+    if (rc) {
+        dpl_sym_release(parsec, A, &s, 0, NULL);
+        return dpl_mi_fail("dplasma_dposv", rc);
+    }
+    mi_last_span_ms(parsec->mi, &ms_f);
+    if (info == 0) {
+        if ((rc = dpl_ge_acquire(parsec, B, 1, &g, &parsec->last.stage_in_ms))) {
+            dpl_sym_release(parsec, A, &s, 1, &parsec->last.stage_out_ms);
+            return rc;
+        }
+        rc = mi_dpotrs_tiles(parsec->mi, s.N, s.NB, s.d, g.N, g.d, g.ld);
+        if (rc) rc = dpl_mi_fail("dplasma_dposv", rc);
+        else mi_last_span_ms(parsec->mi, &ms_s);
+        int r2 = dpl_ge_release(parsec, B, &g, rc == 0, &parsec->last.stage_out_ms);
+        if (!rc) rc = r2;
+    }
+    parsec->last.total_ms = ms_f + ms_s;
+    int r3 = dpl_sym_release(parsec, A, &s, 1, &parsec->last.stage_out_ms);
+    if (rc) return rc;
+    if (r3) return r3;
+    return info;
+}

@@ -1,0 +1,120 @@
+/*
+ * dplasma.h -- DPLASMA's tile Cholesky factor-and-solve entry points, served
+ * by hand-written gfx950 (MI355X) kernels with the tiles resident in HBM.
+ *
+ * Drop-in for the reference prototypes (src/include/dplasma/dplasma_z.h after
+ * the z->d / z->s precision substitution) on a single rank with one GPU.
+ * Each prototype cites the reference declaration it replaces.  Return values
+ * and argument checks follow the reference wrappers (cited at each function
+ * in the implementation); documented deviations:
+ *   - dplasma_dpotrf reports info > 0 for a non-SPD matrix also when the
+ *     factorization ran on the GPU (the reference's HIP/CUDA bodies drop the
+ *     device info, src/zpotrf_L.jdf:135,163);
+ *   - runtime failures (a HIP error, out of HBM) return a negative code
+ *     instead of aborting the run (src/zpotrf_L.jdf:164,284,411,567).
+ */
+#ifndef DPLASMA_MI_DPLASMA_H
+#define DPLASMA_MI_DPLASMA_H
+
+#include "parsec_shim.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- constants (src/include/dplasma/constants.h:31-99) */
+#define DPLASMA_SUCCESS                 0
+#define DPLASMA_ERR_NOT_INITIALIZED  -101
+#define DPLASMA_ERR_NOT_SUPPORTED    -103
+#define DPLASMA_ERR_ILLEGAL_VALUE    -104
+#define DPLASMA_ERR_OUT_OF_RESOURCES -106
+
+typedef int dplasma_enum_t;
+
+#define dplasmaNoTrans       111
+#define dplasmaTrans         112
+#define dplasmaConjTrans     113
+#define dplasmaUpper         121
+#define dplasmaLower         122
+#define dplasmaUpperLower    123
+#define dplasmaNonUnit       131
+#define dplasmaUnit          132
+#define dplasmaLeft          141
+#define dplasmaRight         142
+#define dplasmaOneNorm       171
+#define dplasmaInfNorm       175
+#define dplasmaMaxNorm       177
+
+/* ---------------- Cholesky (dplasma_z.h:81-84,172,239,273) */
+int dplasma_dpotrf(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A);
+int dplasma_dpotrf_rec(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A, int hmb);
+parsec_taskpool_t *dplasma_dpotrf_New(dplasma_enum_t uplo, parsec_tiled_matrix_t *A, int *INFO);
+void dplasma_dpotrf_Destruct(parsec_taskpool_t *o);
+void dplasma_dpotrf_setrecursive(parsec_taskpool_t *o, int hmb);
+int dplasma_dpotrs(parsec_context_t *parsec, dplasma_enum_t uplo,
+                   const parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B);
+int dplasma_dposv(parsec_context_t *parsec, dplasma_enum_t uplo,
+                  parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B);
+
+/* ---------------- triangular solve (dplasma_z.h:51,153,221) */
+int dplasma_dtrsm(parsec_context_t *parsec, dplasma_enum_t side, dplasma_enum_t uplo,
+                  dplasma_enum_t trans, dplasma_enum_t diag, double alpha,
+                  const parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B);
+parsec_taskpool_t *dplasma_dtrsm_New(dplasma_enum_t side, dplasma_enum_t uplo,
+                                     dplasma_enum_t trans, dplasma_enum_t diag, double alpha,
+                                     const parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B);
+void dplasma_dtrsm_Destruct(parsec_taskpool_t *o);
+
+/* ---------------- generators and copies (dplasma_z.h:97,107-109,193-195,260-262) */
+int dplasma_dplgsy(parsec_context_t *parsec, double bump, dplasma_enum_t uplo,
+                   parsec_tiled_matrix_t *A, unsigned long long int seed);
+int dplasma_dplghe(parsec_context_t *parsec, double bump, dplasma_enum_t uplo,
+                   parsec_tiled_matrix_t *A, unsigned long long int seed);
+int dplasma_dplrnt(parsec_context_t *parsec, int diagdom, parsec_tiled_matrix_t *A,
+                   unsigned long long int seed);
+parsec_taskpool_t *dplasma_dplgsy_New(double bump, dplasma_enum_t uplo,
+                                      parsec_tiled_matrix_t *A, unsigned long long int seed);
+void dplasma_dplgsy_Destruct(parsec_taskpool_t *o);
+int dplasma_dlacpy(parsec_context_t *parsec, dplasma_enum_t uplo,
+                   const parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B);
+
+/* ---------------- checks (dplasma_z.h:279-280); computed on the GPU */
+int check_dpotrf(parsec_context_t *parsec, int loud, dplasma_enum_t uplo,
+                 parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *A0);
+int check_daxmb(parsec_context_t *parsec, int loud, dplasma_enum_t uplo,
+                parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *b, parsec_tiled_matrix_t *x);
+
+/* ---------------- data placement (src/dplasmaaux.h:147-151) */
+/* Makes the uplo part of A resident in HBM until parsec_devices_release_memory()
+ * or dplasma_data_release(): operations on A then work on the device copy and
+ * skip the host<->device staging; the host copy is refreshed by
+ * dplasma_data_sync_host() and by the release calls.  The operation/op_args
+ * parameters of the reference (a per-tile device map for multi-GPU) are
+ * accepted and ignored: one GPU holds every tile. */
+typedef void *parsec_tiled_matrix_unary_op_t;
+int dplasma_advise_data_on_device(parsec_context_t *parsec, dplasma_enum_t uplo,
+                                  parsec_tiled_matrix_t *A,
+                                  parsec_tiled_matrix_unary_op_t operation, void *op_args);
+
+/* ================================================================
+ * Extensions of this library (no counterpart in the reference)
+ * ================================================================ */
+
+/* Copies a resident matrix back to its host storage (no-op otherwise). */
+int dplasma_data_sync_host(parsec_context_t *parsec, parsec_tiled_matrix_t *A);
+/* Writes back (if needed) and drops the device copy of A. */
+int dplasma_data_release(parsec_context_t *parsec, parsec_tiled_matrix_t *A);
+
+/* Kernel-level timing of the last blocking call on this context, in ms
+ * (GPU events on the streams the kernels ran on). */
+typedef struct dplasma_mi_timing_s {
+    double total_ms;      /* first kernel start -> last kernel end */
+    double stage_in_ms;   /* host -> device copies */
+    double stage_out_ms;  /* device -> host copies */
+} dplasma_mi_timing_t;
+int dplasma_mi_last_timing(parsec_context_t *parsec, dplasma_mi_timing_t *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,131 @@
+/*
+ * dplasma_mi.h -- thin C ABI of the gfx950 (MI355X) kernel library
+ * libdplasma_mi.so.  The C host code behind dplasma.h reaches HIP only
+ * through these functions: plain pointers (device pointers where named d*),
+ * sizes and an opaque context; every function returns 0 or a negative code
+ * (mi_last_error() has the message).  Blocking functions synchronize the
+ * context's streams before returning.
+ *
+ * Device layouts:
+ *   packed lower tiles  tile (m,n), m >= n, of an MT x MT grid at tile index
+ *                       n*MT - n*(n-1)/2 + (m-n); NB*NB elements per tile,
+ *                       column-major, ld = NB; rows >= N - m*NB of the last
+ *                       tile row are padding and never written.
+ *   column-major        right-hand sides B: N x NRHS, leading dimension ldb.
+ *
+ * Each hot-path entry cites the reference task body or wrapper it replaces.
+ */
+#ifndef DPLASMA_MI_KERNELS_H
+#define DPLASMA_MI_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mi_ctx_s mi_ctx_t;
+
+/* ------------------------------------------------------------ context */
+/* Per-device streams, the device info word and the profiler
+ * (replaces the per-stream handles and the d_iinfo workspace of
+ * src/dplasmaaux_hip.c:43-96 and src/zpotrf_wrapper.c:101-116,237-249). */
+int  mi_ctx_create(int device, mi_ctx_t **ctx);
+void mi_ctx_destroy(mi_ctx_t *ctx);
+int  mi_device_count(int *count);
+const char *mi_last_error(void);
+int  mi_synchronize(mi_ctx_t *ctx);
+
+/* ------------------------------------------------------------ memory */
+int mi_malloc(mi_ctx_t *ctx, void **dptr, size_t bytes);
+int mi_free(mi_ctx_t *ctx, void *dptr);
+int mi_host_alloc(void **hptr, size_t bytes);     /* page-locked host memory */
+int mi_host_free(void *hptr);
+int mi_memcpy_h2d(mi_ctx_t *ctx, void *dst, const void *src, size_t bytes);
+int mi_memcpy_d2h(mi_ctx_t *ctx, void *dst, const void *src, size_t bytes);
+int mi_memcpy_d2d(mi_ctx_t *ctx, void *dst, const void *src, size_t bytes);
+/* 2-D copies (rows of `width` bytes): the pitched stage-in/out of
+ * src/cuda/lapack_cuda_stage_in.c:18-162. */
+int mi_memcpy2d_h2d(mi_ctx_t *ctx, void *dst, size_t dpitch, const void *src, size_t spitch,
+                    size_t width, size_t height);
+int mi_memcpy2d_d2h(mi_ctx_t *ctx, void *dst, size_t dpitch, const void *src, size_t spitch,
+                    size_t width, size_t height);
+int mi_memset(mi_ctx_t *ctx, void *dptr, int value, size_t bytes);
+
+/* ------------------------------------------------------------ layout */
+/* Packed upper tiles (tile (m,n), m <= n, at index n*(n+1)/2 + m) to packed
+ * lower tiles holding the transposes, and back; used for uplo = Upper, where
+ * U = L^T and every kernel below works on L. */
+int mi_dsym_upper_to_lower(mi_ctx_t *ctx, int N, int NB, const double *dU, double *dL);
+int mi_dsym_lower_to_upper(mi_ctx_t *ctx, int N, int NB, const double *dL, double *dU);
+/* General tiles (MB x NB, ld MB, tile (m,n) at n*MT+m) <-> column-major. */
+int mi_dtiles_to_colmajor(mi_ctx_t *ctx, int M, int N, int MB, int NB, const double *dT,
+                          double *dA, int lda);
+int mi_dcolmajor_to_tiles(mi_ctx_t *ctx, int M, int N, int MB, int NB, const double *dA,
+                          int lda, double *dT);
+
+/* ------------------------------------------------------------ generators */
+/* dplasma_dplgsy on packed lower tiles (src/zplgsy_wrapper.c:24-46 ->
+ * src/cores/core_zplgsy.c:72-150), diagonal tiles filled in full. */
+int mi_dplgsy_tiles(mi_ctx_t *ctx, double bump, int N, int NB, double *dT, uint64_t seed);
+/* dplasma_dplrnt into a column-major M x N block at (m0,n0) of a gM-row
+ * matrix (src/zplrnt_wrapper.c:25-45 -> src/cores/core_zplrnt.c:68-91). */
+int mi_dplrnt(mi_ctx_t *ctx, int M, int N, double *dA, int lda, int gM, int m0, int n0,
+              uint64_t seed);
+/* A(i,i) += alpha for i < n (the diagdom option of dplasma_dplrnt,
+ * src/zplrnt_wrapper.c:80-93). */
+int mi_dadd_diag(mi_ctx_t *ctx, int n, double *dA, int lda, double alpha);
+
+/* ------------------------------------------------------------ hot path */
+/* In-place tile Cholesky, lower: the POTRF/TRSM/SYRK/GEMM task classes of
+ * src/zpotrf_L.jdf:91-633 as fused per-step launches with one-panel
+ * look-ahead.  *info receives k*NB + iinfo of the first non-positive pivot
+ * (src/zpotrf_L.jdf:200-202), 0 on success.  Blocking. */
+int mi_dpotrf_tiles(mi_ctx_t *ctx, int N, int NB, double *dT, int *info);
+/* Left/Lower tile triangular solve, B <- alpha op(L)^{-1} B, op = NoTrans
+ * (111) or Trans (112), diag NonUnit (131) / Unit (132):
+ * src/ztrsm_LLN.jdf, src/ztrsm_LLT.jdf.  Blocking. */
+int mi_dtrsm_LL(mi_ctx_t *ctx, int trans, int diag, double alpha, int N, int NB,
+                const double *dT, int NRHS, double *dB, int ldb);
+/* B <- (L L^T)^{-1} B: src/zpotrs_wrapper.c:91-94.  Blocking. */
+int mi_dpotrs_tiles(mi_ctx_t *ctx, int N, int NB, const double *dT, int NRHS, double *dB, int ldb);
+
+/* Single task-class launches (one step k of the factorization), for tests:
+ *   POTRF(k)            src/zpotrf_L.jdf:91-208
+ *   TRSM(k+1..MT-1, k)  src/zpotrf_L.jdf:214-343
+ *   SYRK/GEMM(m,n,k) for tile columns n in [n_begin, n_end)
+ *                       src/zpotrf_L.jdf:349-633 */
+int mi_dpotrf_step_potrf(mi_ctx_t *ctx, int N, int NB, double *dT, int k, int *info);
+int mi_dpotrf_step_trsm(mi_ctx_t *ctx, int N, int NB, double *dT, int k);
+int mi_dpotrf_step_update(mi_ctx_t *ctx, int N, int NB, double *dT, int k, int n_begin, int n_end);
+
+/* ------------------------------------------------------------ checks */
+/* Norms and residuals behind check_dpotrf / check_daxmb
+ * (src/dplasma_zcheck.c:63-133,179-224) computed on the GPU:
+ *   *Anorm = ||A0||_oo (symmetric, lower tiles), *Rnorm = ||L L^T - A0||_oo. */
+int mi_dcheck_potrf(mi_ctx_t *ctx, int N, int NB, const double *dL, const double *dA0,
+                    double *Anorm, double *Rnorm);
+/* ||A0||_oo, ||b||_oo, ||x||_oo and ||b - A0 x||_oo for column-major b, x. */
+int mi_dcheck_axmb(mi_ctx_t *ctx, int N, int NB, const double *dA0, int NRHS,
+                   const double *db, int ldb, const double *dx, int ldx,
+                   double *Anorm, double *Bnorm, double *Xnorm, double *Rnorm);
+
+/* ------------------------------------------------------------ profiler */
+/* When enabled, every kernel launch is bracketed by HIP events recorded on
+ * the stream it runs on; mi_prof_get aggregates per kernel name after the
+ * next synchronization. */
+int mi_prof_enable(mi_ctx_t *ctx, int on);
+int mi_prof_reset(mi_ctx_t *ctx);
+int mi_prof_get(mi_ctx_t *ctx, const char *kernel, int *launches, double *total_ms,
+                double *flops, double *bytes);
+/* Number of distinct kernel names recorded, and the i-th name. */
+int mi_prof_count(mi_ctx_t *ctx);
+const char *mi_prof_name(mi_ctx_t *ctx, int i);
+/* Span of the last blocking hot-path call (first launch to last), ms. */
+int mi_last_span_ms(mi_ctx_t *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
