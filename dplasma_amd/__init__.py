@@ -186,6 +186,7 @@ class Matrix:
         desc.mat = ptr
         self.array = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)),
                                            shape=(nelem,))
+        self.array[:] = 0.0      # deterministic padding for comparisons
 
     @property
     def ptr(self):

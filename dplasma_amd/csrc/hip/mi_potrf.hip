@@ -341,7 +341,8 @@ __global__ __launch_bounds__(256) void k_update_mfma(double *__restrict__ T, int
         __syncthreads();
     }
 
-    // epilogue: lane holds rows 4*(lane/16)+q, column lane%16 of each 16x16 tile
+    // epilogue: element q of lane l holds row 4*q + l/16, column l%16 of each
+    // 16x16 tile (v_mfma_f64_16x16x4_f64 output layout, probed on gfx950)
     const bool diag = (m == n) && (bm == bn);
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -351,7 +352,7 @@ __global__ __launch_bounds__(256) void k_update_mfma(double *__restrict__ T, int
             if (col >= cols_n) continue;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int rr = r_base + wm * 64 + i * 16 + 4 * fk + q;
+                const int rr = r_base + wm * 64 + i * 16 + 4 * q + fk;
                 if (rr < rows_m && (!diag || rr >= col))
                     Cp[(int64_t)col * NB + rr] -= acc[i][j][q];
             }

@@ -67,9 +67,43 @@ static int copy_triangle(parsec_context_t *ctx, const parsec_tiled_matrix_t *A, 
 {
     const int MT = A->lmt, NB = A->mb, N = A->m;
     if (A->dtype == PARSEC_SHIM_SYM_BLOCK_CYCLIC) {
-        void *mat = parsec_shim_mat(A);
+        char *mat = (char *)parsec_shim_mat(A);
         size_t bytes = sym_image_bytes(A);
-        return to_device ? mi_memcpy_h2d(ctx->mi, d, mat, bytes) : mi_memcpy_d2h(ctx->mi, mat, d, bytes);
+        if (to_device) return mi_memcpy_h2d(ctx->mi, d, mat, bytes);
+        if (N % NB == 0) return mi_memcpy_d2h(ctx->mi, mat, d, bytes);
+        /* ragged last tile row: its padding rows stay as the host has them */
+        const size_t tb = (size_t)NB * NB * sizeof(double);
+        const int tm = N - (MT - 1) * NB;
+        for (int n = 0; n < MT; n++) {
+            int64_t first = (uplo == dplasmaLower) ? (int64_t)n * MT - ((int64_t)n * (n - 1)) / 2
+                                                   : ((int64_t)n * (n + 1)) / 2;
+            int ntile = (uplo == dplasmaLower) ? MT - n : n + 1;
+            int rc;
+            if (uplo == dplasmaLower) {
+                /* column n: tiles n..MT-2 are full, tile MT-1 is ragged */
+                if (ntile > 1 && (rc = mi_memcpy_d2h(ctx->mi, mat + first * tb, d + first * (int64_t)NB * NB,
+                                                      (ntile - 1) * tb)))
+                    return rc;
+                int64_t last = first + ntile - 1;
+                int tn = (n == MT - 1) ? tm : NB;
+                if ((rc = mi_memcpy2d_d2h(ctx->mi, mat + last * tb, NB * sizeof(double),
+                                          d + last * (int64_t)NB * NB, NB * sizeof(double),
+                                          tm * sizeof(double), tn)))
+                    return rc;
+            } else {
+                /* column n holds tiles 0..n; only column MT-1 is ragged (in width) */
+                int tn = (n == MT - 1) ? tm : NB;
+                for (int m = 0; m <= n; m++) {
+                    int64_t idx = first + m;
+                    int tr = (m == MT - 1) ? tm : NB;
+                    if ((rc = mi_memcpy2d_d2h(ctx->mi, mat + idx * tb, NB * sizeof(double),
+                                              d + idx * (int64_t)NB * NB, NB * sizeof(double),
+                                              tr * sizeof(double), tn)))
+                        return rc;
+                }
+            }
+        }
+        return 0;
     }
     for (int n = 0; n < MT; n++) {
         int lo = (uplo == dplasmaLower) ? n : 0, hi = (uplo == dplasmaLower) ? MT : n + 1;
