@@ -101,6 +101,7 @@ def libs():
     _sig(M, "mi_memcpy2d_h2d", i, [vp, vp, sz, vp, sz, sz, sz])
     _sig(M, "mi_memcpy2d_d2h", i, [vp, vp, sz, vp, sz, sz, sz])
     _sig(M, "mi_memset", i, [vp, vp, i, sz])
+    _sig(M, "mi_dcopy", i, [vp, sz, vp, vp])
     _sig(M, "mi_dsym_upper_to_lower", i, [vp, i, i, vp, vp])
     _sig(M, "mi_dsym_lower_to_upper", i, [vp, i, i, vp, vp])
     _sig(M, "mi_dtiles_to_colmajor", i, [vp, i, i, i, i, vp, vp, i])
@@ -157,6 +158,7 @@ def libs():
     _sig(D, "dplasma_data_sync_host", i, [vp, vp])
     _sig(D, "dplasma_data_release", i, [vp, vp])
     _sig(D, "dplasma_mi_last_timing", i, [vp, ctypes.POINTER(Timing)])
+    _sig(D, "dplasma_mi_context", vp, [vp])
     return _mi, _dpl
 
 
@@ -337,6 +339,27 @@ class Context:
         t = Timing()
         self.D.dplasma_mi_last_timing(self.ptr, ctypes.byref(t))
         return {"total_ms": t.total_ms, "stage_in_ms": t.stage_in_ms, "stage_out_ms": t.stage_out_ms}
+
+    # -- kernel profiler of the underlying kernel context
+    def prof_enable(self, on=True):
+        mi, _ = libs()
+        mi.mi_prof_enable(self.D.dplasma_mi_context(self.ptr), int(on))
+
+    def prof_reset(self):
+        mi, _ = libs()
+        mi.mi_prof_reset(self.D.dplasma_mi_context(self.ptr))
+
+    def prof(self):
+        """{kernel: (launches, total_ms, algorithmic flops, algorithmic bytes)}"""
+        mi, _ = libs()
+        h = self.D.dplasma_mi_context(self.ptr)
+        out = {}
+        for k in range(mi.mi_prof_count(h)):
+            name = mi.mi_prof_name(h, k)
+            n, ms, fl, by = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            mi.mi_prof_get(h, name, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by))
+            out[name.decode()] = (n.value, ms.value, fl.value, by.value)
+        return out
 
 
 class Device:

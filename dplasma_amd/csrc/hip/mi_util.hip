@@ -316,6 +316,14 @@ __global__ __launch_bounds__(256) void k_max(const double *__restrict__ v, int64
     if (threadIdx.x == 0) *out = sm[0];
 }
 
+__global__ __launch_bounds__(256) void k_dcopy(const double2 *__restrict__ s, double2 *__restrict__ d,
+                                              int64_t n2)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
 __global__ void k_add_diag(double *__restrict__ A, int n, int lda, double alpha)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -378,6 +386,16 @@ int mi_dplrnt(mi_ctx_t *c, int M, int Nc, double *dA, int lda, int gM, int m0, i
     const int64_t work = (int64_t)Nc * ((M + SEG - 1) / SEG);
     MI_LAUNCH(c, c->s_main, "dplrnt", 0.0, (double)M * Nc * 8.0,
               k_dplrnt<<<grid_for(work), 256, 0, c->s_main>>>(dA, M, Nc, lda, gM, m0, n0, seed));
+    return mi_synchronize(c);
+}
+
+int mi_dcopy(mi_ctx_t *c, size_t n, const double *s, double *d)
+{
+    if (n == 0) return 0;
+    if ((n & 1) || ((uintptr_t)s & 15) || ((uintptr_t)d & 15)) { set_error("mi_dcopy: even n, 16B alignment"); return -2; }
+    const int64_t n2 = (int64_t)(n / 2);
+    MI_LAUNCH(c, c->s_main, "dcopy", 0.0, 16.0 * n, k_dcopy<<<2048, 256, 0, c->s_main>>>(
+                  (const double2 *)s, (double2 *)d, n2));
     return mi_synchronize(c);
 }
 

@@ -223,3 +223,5 @@ int dplasma_mi_last_timing(parsec_context_t *parsec, dplasma_mi_timing_t *t)
     *t = parsec->last;
     return 0;
 }
+
+struct mi_ctx_s *dplasma_mi_context(parsec_context_t *parsec) { return parsec->mi; }

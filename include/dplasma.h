@@ -114,6 +114,10 @@ typedef struct dplasma_mi_timing_s {
 } dplasma_mi_timing_t;
 int dplasma_mi_last_timing(parsec_context_t *parsec, dplasma_mi_timing_t *t);
 
+/* The kernel-library context behind a parsec context (for the profiler of
+ * dplasma_mi.h). */
+struct mi_ctx_s *dplasma_mi_context(parsec_context_t *parsec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,9 @@ int mi_memcpy2d_h2d(mi_ctx_t *ctx, void *dst, size_t dpitch, const void *src, si
 int mi_memcpy2d_d2h(mi_ctx_t *ctx, void *dst, size_t dpitch, const void *src, size_t spitch,
                     size_t width, size_t height);
 int mi_memset(mi_ctx_t *ctx, void *dptr, int value, size_t bytes);
+/* Plain streaming copy kernel of n doubles (reads n*8 bytes, writes n*8
+ * bytes): the reference point that scales HBM traffic counters. */
+int mi_dcopy(mi_ctx_t *ctx, size_t n, const double *dsrc, double *ddst);
 
 /* ------------------------------------------------------------ layout */
 /* Packed upper tiles (tile (m,n), m <= n, at index n*(n+1)/2 + m) to packed
