@@ -27,7 +27,8 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TF = 78.6          # MI355X dense FP64 matrix peak, TFLOP/s
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, GB/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dpotrf_update_mfma_traffic.json")
+UPDATE_KERNEL = "dpotrf_update"   # trailing SYRK/GEMM update, the dominant kernel
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dpotrf_update_traffic.json")
 
 
 def flops_dpotrf(n):
@@ -139,7 +140,7 @@ def main():
     assert ctx.dpotrf(dp.LOWER, A) == 0
     prof = ctx.prof()
     ctx.prof_enable(False)
-    upd = prof.get("dpotrf_update_mfma", (0, 0.0, 0.0, 0.0))
+    upd = prof.get(UPDATE_KERNEL, (0, 0.0, 0.0, 0.0))
     achieved = (upd[2] / upd[0]) / (upd[1] / upd[0] * 1e-3) / 1e12 if upd[0] else 0.0
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -185,7 +186,7 @@ def main():
         "kernels_ms": {k: round(v[1], 3) for k, v in prof.items()},
         "roofline": {
             "bound": "mfma",
-            "kernel": "dpotrf_update_mfma",
+            "kernel": UPDATE_KERNEL,
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TF,
             "unit": "TFLOP/s",

@@ -115,6 +115,8 @@ def libs():
     _sig(M, "mi_dpotrf_step_potrf", i, [vp, i, i, vp, i, pi])
     _sig(M, "mi_dpotrf_step_trsm", i, [vp, i, i, vp, i])
     _sig(M, "mi_dpotrf_step_update", i, [vp, i, i, vp, i, i, i])
+    _sig(M, "mi_set_update_engine", i, [i])
+    _sig(M, "mi_set_panel_engine", i, [i])
     _sig(M, "mi_dcheck_potrf", i, [vp, i, i, vp, vp, pd, pd])
     _sig(M, "mi_dcheck_axmb", i, [vp, i, i, vp, i, vp, i, vp, i, pd, pd, pd, pd])
     _sig(M, "mi_prof_enable", i, [vp, i])

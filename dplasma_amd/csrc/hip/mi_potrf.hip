@@ -24,6 +24,8 @@
 // as soon as column k+1 has received its step-k update, so it overlaps the
 // rest of the step-k trailing update (one-panel look-ahead).
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include "mi_common.hpp"
 #include "dplasma_mi.h"
 
@@ -33,6 +35,214 @@ namespace {
 
 constexpr int PJB = 32;            // panel width inside the diagonal factorization
 constexpr int POTRF_THREADS = 512;
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// One wave: acc[ct] += X[r0+i][0:K] * Y[c0+16ct+j][0:K]^T for the 16x16 output
+// tiles ct = 0, 1 (i, j < 16) on v_mfma_f64_16x16x4_f64, operands read straight
+// from column-major global memory (element (r, k) at X[k*ldx + r]); rows at or
+// beyond xr_lim / yr_lim read as zero.  K is a multiple of 32.  Operand lane
+// layout: A(i, k) and B(k, j) sit in lane 16k + i (resp. 16k + j); result
+// element q of lane l is D(4q + l/16, l%16).
+__device__ inline void mfma_panel_2(const double *__restrict__ X, int ldx, int r0, int xr_lim,
+                                    const double *__restrict__ Y, int ldy, int c0, int yr_lim, int K,
+                                    dx4 acc[2])
+{
+    const int lane = threadIdx.x & 63, fr = lane & 15, fk = lane >> 4;
+    const int xr = r0 + fr, y0 = c0 + fr, y1 = c0 + 16 + fr;
+    const bool xv = xr < xr_lim, yv0 = y0 < yr_lim, yv1 = y1 < yr_lim;
+    dx4 e0 = {0, 0, 0, 0}, e1 = e0, o0 = e0, o1 = e0;
+    // 32 columns of K per pass: all 24 operand loads are issued before the
+    // 16 MFMAs that consume them
+    for (int k = 0; k < K; k += 32) {
+        double a[8], b0[8], b1[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int64_t kk = k + 4 * s + fk;
+            a[s] = xv ? X[kk * ldx + xr] : 0.0;
+            b0[s] = yv0 ? Y[kk * ldy + y0] : 0.0;
+            b1[s] = yv1 ? Y[kk * ldy + y1] : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < 8; s += 2) {
+            e0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b0[s], e0, 0, 0, 0);
+            e1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b1[s], e1, 0, 0, 0);
+            o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s + 1], b0[s + 1], o0, 0, 0, 0);
+            o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s + 1], b1[s + 1], o1, 0, 0, 0);
+        }
+    }
+    acc[0] += e0 + o0;
+    acc[1] += e1 + o1;
+}
+
+// ---------------------------------------------------------------------------
+// POTRF(k), left-looking by 32-column panels in one workgroup of 8 waves:
+//   P = A[j0:n, j0:j0+32] - L[j0:n, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, LDS P)
+//   L_jj = chol(P[0:32])       wave 0, one row per lane in registers
+//   W_jj = L_jj^{-1}           32 lanes, one column each (also to Winv)
+//   P[32:] <- P[32:] W_jj^T    one row per thread
+// then the panel is written back.  info = k*NB + j of the first non-positive
+// pivot j (1-based), as CORE_dpotrf reports it (src/zpotrf_L.jdf:200-202).
+__global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int n, int ld, int RC,
+                                                   int info_base, int *__restrict__ d_info,
+                                                   double *__restrict__ Winv)
+{
+    extern __shared__ double smem[];
+    double *P = smem;               // 32 columns x RC rows
+    double *Ws = P + 32 * RC;       // W(i, j) at Ws[j*32 + i]
+    double *colb = Ws + 1024;       // one column of L_jj
+    __shared__ int s_fail;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int fr = lane & 15, fk = lane >> 4;
+    if (tid == 0) s_fail = 0;
+    __syncthreads();
+
+    for (int j0 = 0; j0 < n; j0 += 32) {
+        const int jw = min(32, n - j0), R = n - j0, R16 = (R + 15) >> 4;
+        double *Aj = A + j0;                     // row j0 of column 0
+        for (int rt = wave; rt < R16; rt += 8) {
+            dx4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+            if (j0 > 0) mfma_panel_2(Aj, ld, rt * 16, R, Aj, ld, 0, jw, j0, acc);
+#pragma unroll
+            for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int r = rt * 16 + 4 * q + fk, c = ct * 16 + fr;
+                    P[c * RC + r] = (r < R && c < jw) ? Aj[(int64_t)(j0 + c) * ld + r] - acc[ct][q] : 0.0;
+                }
+        }
+        __syncthreads();
+
+        if (wave == 0) {
+            double row[32];
+            bool failed = false;
+#pragma unroll
+            for (int c = 0; c < 32; c++) row[c] = (lane < 32) ? P[c * RC + lane] : 0.0;
+#pragma unroll
+            for (int c = 0; c < 32; c++) {
+                if (c < jw && !failed) {
+                    const double d0 = __shfl(row[c], c, 64);
+                    if (!(d0 > 0.0)) {
+                        if (lane == 0) s_fail = j0 + c + 1;
+                        failed = true;       // uniform: d0 is the same in every lane
+                        continue;
+                    }
+                    const double d = sqrt(d0);
+                    const double l = row[c] / d;
+                    if (lane == c) row[c] = d;
+                    else if (lane > c) row[c] = l;
+                    if (lane < 32) colb[lane] = (lane > c) ? l : 0.0;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (lane > c) {
+#pragma unroll
+                        for (int c2 = c + 1; c2 < 32; c2++) row[c2] -= l * colb[c2];
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+            if (lane < 32) {
+#pragma unroll
+                for (int c = 0; c < 32; c++)
+                    if (c <= lane) P[c * RC + lane] = row[c];
+            }
+        }
+        __syncthreads();
+        if (s_fail) break;
+
+        if (tid < 32) {
+            const int j = tid;
+            double w[32];
+#pragma unroll
+            for (int i = 0; i < 32; i++) {
+                if (i < jw) {
+                    double s = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+                    for (int p = 0; p < 32; p++)
+                        if (p < i) s -= P[p * RC + i] * w[p];
+                    w[i] = (i >= j) ? s / P[i * RC + i] : 0.0;
+                } else {
+                    w[i] = (i == j) ? 1.0 : 0.0;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 32; i++) Ws[j * 32 + i] = w[i];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < 1024; idx += 512) Winv[(j0 >> 5) * 1024 + idx] = Ws[idx];
+
+        for (int r = 32 + tid; r < R; r += 512) {
+            double b[32];
+#pragma unroll
+            for (int c = 0; c < 32; c++) b[c] = P[c * RC + r];
+#pragma unroll
+            for (int c = 0; c < 32; c++) {
+                double s = 0.0;
+#pragma unroll
+                for (int p = 0; p < 32; p++)
+                    if (p <= c) s += b[p] * Ws[p * 32 + c];
+                if (c < jw) P[c * RC + r] = s;
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < jw * R; idx += 512) {
+            const int c = idx / R, r = idx - c * R;
+            if (r >= c) Aj[(int64_t)(j0 + c) * ld + r] = P[c * RC + r];
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && s_fail) atomicCAS(d_info, 0, info_base + s_fail);
+}
+
+// ---------------------------------------------------------------------------
+// TRSM(m,k) for m = k+1 .. MT-1: C <- C L^{-T}, one workgroup (4 waves) per
+// 64-row strip of a tile, 32 columns at a time:
+//   P = C[:, j0:j0+32] - X[:, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, wave w: rows 16w..)
+//   X[:, j0:j0+32] = P W_jj^T                              (W from POTRF(k))
+constexpr int TS_ROWS = 64;
+__global__ __launch_bounds__(256) void k_trsm_tile(double *__restrict__ T, int N, int NB, int MT, int k,
+                                                  const double *__restrict__ Winv)
+{
+    __shared__ double P[32][TS_ROWS + 1];
+    __shared__ double Ws[1024];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int fr = lane & 15, fk = lane >> 4;
+    const int strips = (NB + TS_ROWS - 1) / TS_ROWS;
+    const int m = k + 1 + blockIdx.x / strips;
+    const int r0 = (blockIdx.x % strips) * TS_ROWS;
+    const int rows_m = tile_rows(N, NB, MT, m);
+    if (r0 >= rows_m) return;
+    const int rv = min(TS_ROWS, rows_m - r0);
+    double *C = T + tile_index(MT, m, k) * (int64_t)NB * NB + r0;
+    const double *L = T + tile_index(MT, k, k) * (int64_t)NB * NB;
+    const int r = tid & 63, cg = tid >> 6;
+
+    for (int j0 = 0; j0 < NB; j0 += 32) {
+        const int jw = min(32, NB - j0);
+        dx4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        if (j0 > 0) mfma_panel_2(C, NB, wave * 16, rv, L + j0, NB, 0, jw, j0, acc);
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = wave * 16 + 4 * q + fk, c = ct * 16 + fr;
+                P[c][rr] = (rr < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + rr] - acc[ct][q] : 0.0;
+            }
+        for (int idx = tid; idx < 1024; idx += 256) Ws[idx] = Winv[(j0 >> 5) * 1024 + idx];
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++) {
+            const int c = cg * 8 + jj;
+            double s = 0.0;
+            for (int p = 0; p <= c; p++) s += P[p][r] * Ws[p * 32 + c];
+            if (r < rv && c < jw) C[(int64_t)(j0 + c) * NB + r] = s;
+        }
+        __syncthreads();
+    }
+}
 
 // ---------------------------------------------------------------------------
 // POTRF(k): factor the n x n diagonal tile A (ld) in place, lower.
@@ -253,10 +463,28 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double *__restrict__ T, int 
 // [n_begin, n_end), rows m in [n, MT).  One workgroup = one 128x128 block of
 // one tile; four waves, each a 64x64 quadrant as 4x4 MFMA 16x16 tiles.
 constexpr int BM = 128, BK = 16;
-typedef double dx4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_update_mfma(double *__restrict__ T, int N, int NB, int MT, int k,
-                                                    int n_begin, int nsub)
+// workgroup L of an update launch -> tile (m, n) and its 128x128 block
+// (bm, bn); tiles enumerated column by column from column n_begin
+__device__ inline void decode_block(int L, int nsub, int n_begin, int MT, int *m, int *n, int *bm,
+                                    int *bn)
+{
+    const int per_tile = nsub * nsub;
+    int t = L / per_tile;
+    const int s = L - t * per_tile;
+    int nn = n_begin;
+    while (t >= MT - nn) { t -= MT - nn; nn++; }
+    *n = nn;
+    *m = nn + t;
+    *bm = s / nsub;
+    *bn = s - (*bm) * nsub;
+}
+
+// The FP64 MFMA pipe of a gfx950 SIMD needs two resident waves to stay
+// busy (one wave alone issues at half rate), so the kernel is held to 256
+// registers per lane (128 accumulators) and two workgroups share each CU.
+__global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, int N, int NB, int MT, int k,
+                                                       int n_begin, int nsub)
 {
     __shared__ double As[2][BK][BM];
     __shared__ double Bs[2][BK][BM];
@@ -270,19 +498,15 @@ __global__ __launch_bounds__(256) void k_update_mfma(double *__restrict__ T, int
     const int G8 = (G / 8) * 8;
     const int L = (bid < G8) ? (bid % 8) * (G8 / 8) + bid / 8 : bid;
 
-    const int per_tile = nsub * nsub;
-    int t = L / per_tile;
-    const int s = L - t * per_tile;
-    int n = n_begin;
-    while (t >= MT - n) { t -= MT - n; n++; }
-    const int m = n + t;
-    const int bm = s / nsub, bn = s - bm * nsub;
+    int m, n, bm, bn;
+    decode_block(L, nsub, n_begin, MT, &m, &n, &bm, &bn);
     if (m == n && bm < bn) return;                 // strictly upper blocks of a diagonal tile
 
     const int rows_m = tile_rows(N, NB, MT, m);
     const int cols_n = tile_rows(N, NB, MT, n);
     const int r_base = bm * BM, c_base = bn * BM;
     if (r_base >= rows_m || c_base >= cols_n) return;
+    const bool full = (r_base + BM <= rows_m) && (c_base + BM <= cols_n) && (NB % BK == 0) && !(NB & 1);
 
     const double *Ap = T + tile_index(MT, m, k) * (int64_t)NB * NB;
     const double *Bp = T + tile_index(MT, n, k) * (int64_t)NB * NB;
@@ -294,25 +518,45 @@ __global__ __launch_bounds__(256) void k_update_mfma(double *__restrict__ T, int
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
 
-    // global -> register staging: element e = tid + 256*i, kk = e / 128, r = e % 128
-    double ra[8], rb[8];
+    // global -> register staging in 16-byte pairs: pair e = tid + 256*i,
+    // k row kk = e / 64, rows 2*(e % 64) and 2*(e % 64) + 1
+    double2 ra[4], rb[4];
     auto gload = [&](int k0) {
+        if (full) {
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int e = tid + 256 * i;
-            const int kk = e >> 7, r = e & 127;
-            const int kg = k0 + kk;
-            const int ra_row = r_base + r, rb_row = c_base + r;
-            ra[i] = (kg < NB && ra_row < rows_m) ? Ap[(int64_t)kg * NB + ra_row] : 0.0;
-            rb[i] = (kg < NB && rb_row < cols_n) ? Bp[(int64_t)kg * NB + rb_row] : 0.0;
+            for (int i = 0; i < 4; i++) {
+                const int e = tid + 256 * i;
+                const int kk = e >> 6, r = (e & 63) * 2;
+                ra[i] = *(const double2 *)(Ap + (int64_t)(k0 + kk) * NB + r_base + r);
+                rb[i] = *(const double2 *)(Bp + (int64_t)(k0 + kk) * NB + c_base + r);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int e = tid + 256 * i;
+                const int kk = e >> 6, r = (e & 63) * 2;
+                const int kg = k0 + kk;
+                double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+                if (kg < NB) {
+                    const double *ac = Ap + (int64_t)kg * NB;
+                    const double *bc = Bp + (int64_t)kg * NB;
+                    if (r_base + r < rows_m) a0 = ac[r_base + r];
+                    if (r_base + r + 1 < rows_m) a1 = ac[r_base + r + 1];
+                    if (c_base + r < cols_n) b0 = bc[c_base + r];
+                    if (c_base + r + 1 < cols_n) b1 = bc[c_base + r + 1];
+                }
+                ra[i] = make_double2(a0, a1);
+                rb[i] = make_double2(b0, b1);
+            }
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < 4; i++) {
             const int e = tid + 256 * i;
-            As[buf][e >> 7][e & 127] = ra[i];
-            Bs[buf][e >> 7][e & 127] = rb[i];
+            const int kk = e >> 6, r = (e & 63) * 2;
+            *(double2 *)&As[buf][kk][r] = ra[i];
+            *(double2 *)&Bs[buf][kk][r] = rb[i];
         }
     };
 
@@ -359,6 +603,139 @@ __global__ __launch_bounds__(256) void k_update_mfma(double *__restrict__ T, int
         }
 }
 
+// ---------------------------------------------------------------------------
+// The same trailing update on the FP64 VALU.  On gfx950 v_fma_f64 reaches
+// the 78.6 TF FP64 peak while v_mfma_f64_16x16x4_f64 tops out near half of
+// it, so this is the default path.  Work decomposition as k_update_mfma (one
+// 128x128 block of one tile per workgroup, XCD-grouped); thread (tx, ty) =
+// (t % 16, t / 16) keeps the 8x8 block of rows {32g + 2tx + i} and columns
+// {32h + 2ty + j} (g, h < 4, i, j < 2) in registers.  Every k step reads
+// four 16-byte A pairs (16 distinct per wave, contiguous: no bank conflict)
+// and four B pairs (4 distinct per wave: broadcast) for 64 FMAs.
+
+__global__ __launch_bounds__(256) void k_update_valu(double *__restrict__ T, int N, int NB, int MT, int k,
+                                                    int n_begin, int nsub)
+{
+    __shared__ double As[2][BK][BM];
+    __shared__ double Bs[2][BK][BM];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int G8 = (G / 8) * 8;
+    const int L = (bid < G8) ? (bid % 8) * (G8 / 8) + bid / 8 : bid;
+    int m, n, bm, bn;
+    decode_block(L, nsub, n_begin, MT, &m, &n, &bm, &bn);
+    if (m == n && bm < bn) return;
+    const int rows_m = tile_rows(N, NB, MT, m);
+    const int cols_n = tile_rows(N, NB, MT, n);
+    const int r_base = bm * BM, c_base = bn * BM;
+    if (r_base >= rows_m || c_base >= cols_n) return;
+    const bool full = (r_base + BM <= rows_m) && (c_base + BM <= cols_n) && (NB % BK == 0) && !(NB & 1);
+
+    const double *Ap = T + tile_index(MT, m, k) * (int64_t)NB * NB;
+    const double *Bp = T + tile_index(MT, n, k) * (int64_t)NB * NB;
+    double *Cp = T + tile_index(MT, m, n) * (int64_t)NB * NB;
+
+    double acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] = 0.0;
+
+    double2 ra[4], rb[4];
+    auto gload = [&](int k0) {
+        if (full) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int e = tid + 256 * i;
+                const int kk = e >> 6, r = (e & 63) * 2;
+                ra[i] = *(const double2 *)(Ap + (int64_t)(k0 + kk) * NB + r_base + r);
+                rb[i] = *(const double2 *)(Bp + (int64_t)(k0 + kk) * NB + c_base + r);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int e = tid + 256 * i;
+                const int kk = e >> 6, r = (e & 63) * 2;
+                const int kg = k0 + kk;
+                double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+                if (kg < NB) {
+                    const double *ac = Ap + (int64_t)kg * NB;
+                    const double *bc = Bp + (int64_t)kg * NB;
+                    if (r_base + r < rows_m) a0 = ac[r_base + r];
+                    if (r_base + r + 1 < rows_m) a1 = ac[r_base + r + 1];
+                    if (c_base + r < cols_n) b0 = bc[c_base + r];
+                    if (c_base + r + 1 < cols_n) b1 = bc[c_base + r + 1];
+                }
+                ra[i] = make_double2(a0, a1);
+                rb[i] = make_double2(b0, b1);
+            }
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int e = tid + 256 * i;
+            const int kk = e >> 6, r = (e & 63) * 2;
+            *(double2 *)&As[buf][kk][r] = ra[i];
+            *(double2 *)&Bs[buf][kk][r] = rb[i];
+        }
+    };
+
+    const int nstage = (NB + BK - 1) / BK;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int st = 0; st < nstage; st++) {
+        const int buf = st & 1;
+        if (st + 1 < nstage) gload((st + 1) * BK);
+#pragma unroll 2
+        for (int kk = 0; kk < BK; kk++) {
+            double a[8], b[8];
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const double2 va = *(const double2 *)&As[buf][kk][32 * g + 2 * tx];
+                const double2 vb = *(const double2 *)&Bs[buf][kk][32 * g + 2 * ty];
+                a[2 * g] = va.x;
+                a[2 * g + 1] = va.y;
+                b[2 * g] = vb.x;
+                b[2 * g + 1] = vb.y;
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        }
+        if (st + 1 < nstage) lstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    const bool diag = (m == n) && (bm == bn);
+#pragma unroll
+    for (int h = 0; h < 4; h++)
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+            const int col = c_base + 32 * h + 2 * ty + jj;
+            double *cc = Cp + (int64_t)col * NB;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int row = r_base + 32 * g + 2 * tx;
+                if (full && !diag) {
+                    double2 c = *(double2 *)(cc + row);
+                    c.x -= acc[2 * g][2 * h + jj];
+                    c.y -= acc[2 * g + 1][2 * h + jj];
+                    *(double2 *)(cc + row) = c;
+                } else if (col < cols_n) {
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const int rr = row + i;
+                        if (rr < rows_m && (!diag || rr >= col)) cc[rr] -= acc[2 * g + i][2 * h + jj];
+                    }
+                }
+            }
+        }
+}
+
 __global__ void k_fill_int(int *p, int v, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -371,8 +748,15 @@ struct Plan {
     size_t potrf_lds;
 };
 
+// panel engine: 0 = VALU diagonal factor / solve, 1 = MFMA panel updates (default)
+int g_panel_kernel = 1;
+
 int potrf_lds_bytes(int NB, int *RC)
 {
+    if (g_panel_kernel == 1) {
+        *RC = (NB + 15) & ~15;
+        return (32 * (*RC) + 1024 + 32) * (int)sizeof(double);
+    }
     *RC = ((NB + 3) & ~3) + 4;
     return (PJB * (*RC) + 1024) * (int)sizeof(double);
 }
@@ -405,8 +789,12 @@ int launch_potrf(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k
     const int n = tile_rows(N, NB, MT, k);
     double *A = dT + tile_index(MT, k, k) * (int64_t)NB * NB;
     const double fl = (double)n * n * n / 3.0;
-    MI_LAUNCH(c, s, "dpotrf_potrf_diag", fl, 2.0 * n * n * 8.0,
-              k_potrf_diag<<<1, POTRF_THREADS, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
+    if (g_panel_kernel == 1)
+        MI_LAUNCH(c, s, "dpotrf_potrf_diag", fl, 2.0 * n * n * 8.0,
+                  k_potrf_tile<<<1, 512, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
+    else
+        MI_LAUNCH(c, s, "dpotrf_potrf_diag", fl, 2.0 * n * n * 8.0,
+                  k_potrf_diag<<<1, POTRF_THREADS, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
     return 0;
 }
 
@@ -414,13 +802,22 @@ int launch_trsm(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k,
 {
     const int ntile = MT - k - 1;
     if (ntile <= 0) return 0;
-    const int strips = (NB + TRSM_ROWS - 1) / TRSM_ROWS;
     double fl = 0.0;
     for (int m = k + 1; m < MT; m++) fl += (double)tile_rows(N, NB, MT, m) * NB * NB;
-    MI_LAUNCH(c, s, "dpotrf_trsm_panel", fl, fl / NB * 16.0,
-              k_trsm_panel<<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+    if (g_panel_kernel == 1) {
+        const int strips = (NB + TS_ROWS - 1) / TS_ROWS;
+        MI_LAUNCH(c, s, "dpotrf_trsm_panel", fl, fl / NB * 16.0,
+                  k_trsm_tile<<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+    } else {
+        const int strips = (NB + TRSM_ROWS - 1) / TRSM_ROWS;
+        MI_LAUNCH(c, s, "dpotrf_trsm_panel", fl, fl / NB * 16.0,
+                  k_trsm_panel<<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+    }
     return 0;
 }
+
+// trailing-update engine: 0 = FP64 VALU, 1 = FP64 MFMA (default)
+int g_update_kernel = -1;
 
 int launch_update(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, int n_begin,
                   int n_end)
@@ -432,9 +829,18 @@ int launch_update(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int 
     const int64_t grid = ntiles * nsub * nsub;
     if (grid <= 0) return 0;
     if (grid > 0x7fffffff) { set_error("update grid too large"); return -3; }
-    MI_LAUNCH(c, s, "dpotrf_update_mfma", update_flops(N, NB, MT, n_begin, n_end),
-              update_bytes(N, NB, MT, n_begin, n_end),
-              k_update_mfma<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, nsub));
+    if (g_update_kernel < 0) {
+        const char *e = getenv("DPLASMA_MI_UPDATE");
+        g_update_kernel = (e && !strcmp(e, "valu")) ? 0 : 1;
+    }
+    if (g_update_kernel == 1)
+        MI_LAUNCH(c, s, "dpotrf_update", update_flops(N, NB, MT, n_begin, n_end),
+                  update_bytes(N, NB, MT, n_begin, n_end),
+                  k_update_mfma<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, nsub));
+    else
+        MI_LAUNCH(c, s, "dpotrf_update_valu", update_flops(N, NB, MT, n_begin, n_end),
+                  update_bytes(N, NB, MT, n_begin, n_end),
+                  k_update_valu<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, nsub));
     return 0;
 }
 
@@ -454,7 +860,7 @@ int prepare_potrf(int NB)
         return -4;
     }
     if (lds > 65536)
-        MI_CHECK(hipFuncSetAttribute((const void *)k_potrf_diag,
+        MI_CHECK(hipFuncSetAttribute(g_panel_kernel == 1 ? (const void *)k_potrf_tile : (const void *)k_potrf_diag,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     return 0;
 }
@@ -479,6 +885,20 @@ int ensure_ws(size_t bytes)
 }  // namespace
 
 extern "C" {
+
+int mi_set_panel_engine(int which)
+{
+    if (which != 0 && which != 1) { set_error("panel engine must be 0 (VALU) or 1 (MFMA)"); return -2; }
+    g_panel_kernel = which;
+    return 0;
+}
+
+int mi_set_update_engine(int which)
+{
+    if (which != 0 && which != 1) { set_error("update engine must be 0 (VALU) or 1 (MFMA)"); return -2; }
+    g_update_kernel = which;
+    return 0;
+}
 
 int mi_dpotrf_step_potrf(mi_ctx_t *c, int N, int NB, double *dT, int k, int *info)
 {

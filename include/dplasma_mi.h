@@ -100,6 +100,13 @@ int mi_dpotrs_tiles(mi_ctx_t *ctx, int N, int NB, const double *dT, int NRHS, do
  *   SYRK/GEMM(m,n,k) for tile columns n in [n_begin, n_end)
  *                       src/zpotrf_L.jdf:349-633 */
 int mi_dpotrf_step_potrf(mi_ctx_t *ctx, int N, int NB, double *dT, int k, int *info);
+/* Engine of the SYRK/GEMM trailing update: 0 = FP64 VALU FMA, 1 = FP64
+ * MFMA 16x16x4 (default; two resident waves per SIMD keep it at full rate).
+ * Process-wide. */
+int mi_set_update_engine(int which);
+/* Engine of POTRF(k) and the panel TRSM: 0 = VALU kernels, 1 = MFMA panel
+ * updates with register/LDS diagonal blocks (default).  Process-wide. */
+int mi_set_panel_engine(int which);
 int mi_dpotrf_step_trsm(mi_ctx_t *ctx, int N, int NB, double *dT, int k);
 int mi_dpotrf_step_update(mi_ctx_t *ctx, int N, int NB, double *dT, int k, int n_begin, int n_end);
 

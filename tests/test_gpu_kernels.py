@@ -47,8 +47,15 @@ def test_dplrnt_bit_exact_and_golden(dev, po):
         assert np.array_equal(got, po.dplrnt_full(M, nr, seed))
 
 
+@pytest.fixture(params=[0, 1], ids=["panel_valu", "panel_mfma"])
+def panel(request, dev):
+    dev.check(dev.M.mi_set_panel_engine(request.param), "panel engine")
+    yield request.param
+    dev.M.mi_set_panel_engine(1)
+
+
 @pytest.mark.parametrize("n", [1, 31, 32, 33, 93, 256, 320, 512])
-def test_potrf_diag_tile(dev, po, n):
+def test_potrf_diag_tile(dev, po, n, panel):
     A0 = po.dplgsy_full(n, float(n), 3872)
     T = np.asfortranarray(A0).reshape(-1, order="F").copy()
     d = dev.upload(T)
@@ -65,7 +72,7 @@ def test_potrf_diag_tile(dev, po, n):
 
 
 @pytest.mark.parametrize("n,bad", [(64, 0), (64, 40), (256, 100), (93, 92)])
-def test_potrf_diag_info(dev, po, n, bad):
+def test_potrf_diag_info(dev, po, n, bad, panel):
     A0 = po.dplgsy_full(n, float(n), 3872)
     A0[bad, bad] = -5.0 * n
     d = dev.upload(np.asfortranarray(A0).reshape(-1, order="F"))
@@ -76,7 +83,7 @@ def test_potrf_diag_info(dev, po, n, bad):
 
 
 @pytest.mark.parametrize("N,NB", [(378, 93), (640, 128), (1024, 512), (700, 320)])
-def test_panel_trsm(dev, po, N, NB):
+def test_panel_trsm(dev, po, N, NB, panel):
     # after POTRF(0) on GPU, TRSM(1..MT-1, 0) against the oracle tile TRSM
     T = po.dplgsy_symtiles(N, NB, float(N), 3872)
     d = dev.upload(T)
@@ -102,14 +109,19 @@ def test_panel_trsm(dev, po, N, NB):
         assert np.array_equal(g[tm:], C[tm:])
 
 
-@pytest.mark.parametrize("N,NB", [(378, 93), (1024, 256), (1536, 512), (700, 128)])
-def test_trailing_update_mfma(dev, po, N, NB):
+@pytest.mark.parametrize("engine", [0, 1], ids=["valu", "mfma"])
+@pytest.mark.parametrize("N,NB", [(378, 93), (1024, 256), (1536, 512), (700, 128), (1000, 320)])
+def test_trailing_update(dev, po, N, NB, engine):
     rng = np.random.default_rng(N)
     MT = (N + NB - 1) // NB
     T = rng.standard_normal(n_sym(N, NB))
     d = dev.upload(T)
     # whole trailing matrix of step 0
-    dev.check(dev.M.mi_dpotrf_step_update(dev.h, N, NB, d, 0, 1, MT), "update")
+    dev.check(dev.M.mi_set_update_engine(engine), "engine")
+    try:
+        dev.check(dev.M.mi_dpotrf_step_update(dev.h, N, NB, d, 0, 1, MT), "update")
+    finally:
+        dev.M.mi_set_update_engine(1)
     got = dev.download(d, T.shape)
     dev.free(d)
 
