@@ -52,17 +52,23 @@ __device__ inline void mfma_panel_2(const double *__restrict__ X, int ldx, int r
     const int xr = r0 + fr, y0 = c0 + fr, y1 = c0 + 16 + fr;
     const bool xv = xr < xr_lim, yv0 = y0 < yr_lim, yv1 = y1 < yr_lim;
     dx4 e0 = {0, 0, 0, 0}, e1 = e0, o0 = e0, o1 = e0;
-    // 32 columns of K per pass: all 24 operand loads are issued before the
-    // 16 MFMAs that consume them
-    for (int k = 0; k < K; k += 32) {
-        double a[8], b0[8], b1[8];
+    // 32 columns of K per pass; the operands of pass p+1 are in flight while
+    // the 16 MFMAs of pass p run
+    double a[8], b0[8], b1[8];
+    auto load = [&](int k, double *pa, double *pb0, double *pb1) {
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int64_t kk = k + 4 * s + fk;
-            a[s] = xv ? X[kk * ldx + xr] : 0.0;
-            b0[s] = yv0 ? Y[kk * ldy + y0] : 0.0;
-            b1[s] = yv1 ? Y[kk * ldy + y1] : 0.0;
+            pa[s] = xv ? X[kk * ldx + xr] : 0.0;
+            pb0[s] = yv0 ? Y[kk * ldy + y0] : 0.0;
+            pb1[s] = yv1 ? Y[kk * ldy + y1] : 0.0;
         }
+    };
+    if (K > 0) load(0, a, b0, b1);
+    for (int k = 0; k < K; k += 32) {
+        double na[8], nb0[8], nb1[8];
+        const bool more = k + 32 < K;
+        if (more) load(k + 32, na, nb0, nb1);
 #pragma unroll
         for (int s = 0; s < 8; s += 2) {
             e0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b0[s], e0, 0, 0, 0);
@@ -70,27 +76,42 @@ __device__ inline void mfma_panel_2(const double *__restrict__ X, int ldx, int r
             o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s + 1], b0[s + 1], o0, 0, 0, 0);
             o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s + 1], b1[s + 1], o1, 0, 0, 0);
         }
+        if (more) {
+#pragma unroll
+            for (int s = 0; s < 8; s++) { a[s] = na[s]; b0[s] = nb0[s]; b1[s] = nb1[s]; }
+        }
     }
     acc[0] += e0 + o0;
     acc[1] += e1 + o1;
 }
 
 // ---------------------------------------------------------------------------
+// Wave-uniform copy of a double held by one lane (two v_readlane_b32).
+__device__ inline double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // POTRF(k), left-looking by 32-column panels in one workgroup of 8 waves:
 //   P = A[j0:n, j0:j0+32] - L[j0:n, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, LDS P)
-//   L_jj = chol(P[0:32])       wave 0, one row per lane in registers
-//   W_jj = L_jj^{-1}           32 lanes, one column each (also to Winv)
-//   P[32:] <- P[32:] W_jj^T    one row per thread
-// then the panel is written back.  info = k*NB + j of the first non-positive
-// pivot j (1-based), as CORE_dpotrf reports it (src/zpotrf_L.jdf:200-202).
+//   L_jj = chol(P[0:32]), W_jj = L_jj^{-1}   wave 0, one row (resp. one
+//                                            column of W) per lane, values
+//                                            broadcast with v_readlane: no
+//                                            LDS round trip on the serial path
+//   P[32:] <- P[32:] W_jj^T                  MFMA, K = 32
+// then the panel is written back.  W_jj goes to Winv for the panel TRSM.
+// info = k*NB + j of the first non-positive pivot j (1-based), as
+// CORE_dpotrf reports it (src/zpotrf_L.jdf:200-202).
 __global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int n, int ld, int RC,
                                                    int info_base, int *__restrict__ d_info,
                                                    double *__restrict__ Winv)
 {
     extern __shared__ double smem[];
-    double *P = smem;               // 32 columns x RC rows
+    double *P = smem;               // 32 columns x RC rows (RC odd: no bank aliasing)
     double *Ws = P + 32 * RC;       // W(i, j) at Ws[j*32 + i]
-    double *colb = Ws + 1024;       // one column of L_jj
     __shared__ int s_fail;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
@@ -121,70 +142,83 @@ __global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int 
 #pragma unroll
             for (int c = 0; c < 32; c++) {
                 if (c < jw && !failed) {
-                    const double d0 = __shfl(row[c], c, 64);
+                    const double d0 = readlane_d(row[c], c);
                     if (!(d0 > 0.0)) {
                         if (lane == 0) s_fail = j0 + c + 1;
-                        failed = true;       // uniform: d0 is the same in every lane
+                        failed = true;       // uniform: d0 is wave-uniform
                         continue;
                     }
                     const double d = sqrt(d0);
-                    const double l = row[c] / d;
-                    if (lane == c) row[c] = d;
-                    else if (lane > c) row[c] = l;
-                    if (lane < 32) colb[lane] = (lane > c) ? l : 0.0;
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    if (lane > c) {
+                    const double l = row[c] * (1.0 / d);
+                    row[c] = (lane == c) ? d : ((lane > c) ? l : row[c]);
 #pragma unroll
-                        for (int c2 = c + 1; c2 < 32; c2++) row[c2] -= l * colb[c2];
+                    for (int c2 = c + 1; c2 < 32; c2++) {
+                        const double lc2 = readlane_d(l, c2);    // L(c2, c)
+                        if (lane > c) row[c2] -= l * lc2;
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
             if (lane < 32) {
 #pragma unroll
                 for (int c = 0; c < 32; c++)
                     if (c <= lane) P[c * RC + lane] = row[c];
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (!failed) {
+                // W = L^{-1}: lane j builds column j by the column sweep,
+                // column c of L read from LDS as one batch per step
+                double w[32];
+#pragma unroll
+                for (int i = 0; i < 32; i++) w[i] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+                for (int c = 0; c < 32; c++) {
+                    if (c < jw) {
+                        const double *Lc = P + c * RC;
+                        w[c] *= 1.0 / Lc[c];
+#pragma unroll
+                        for (int r = c + 1; r < 32; r++)
+                            if (r < jw) w[r] -= Lc[r] * w[c];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);   // keep each step's loads in its step
+                }
+                if (lane < 32) {
+#pragma unroll
+                    for (int c = 0; c < 32; c++) Ws[lane * 32 + c] = w[c];
+                }
+            }
         }
         __syncthreads();
         if (s_fail) break;
-
-        if (tid < 32) {
-            const int j = tid;
-            double w[32];
-#pragma unroll
-            for (int i = 0; i < 32; i++) {
-                if (i < jw) {
-                    double s = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-                    for (int p = 0; p < 32; p++)
-                        if (p < i) s -= P[p * RC + i] * w[p];
-                    w[i] = (i >= j) ? s / P[i * RC + i] : 0.0;
-                } else {
-                    w[i] = (i == j) ? 1.0 : 0.0;
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 32; i++) Ws[j * 32 + i] = w[i];
-        }
-        __syncthreads();
         for (int idx = tid; idx < 1024; idx += 512) Winv[(j0 >> 5) * 1024 + idx] = Ws[idx];
 
-        for (int r = 32 + tid; r < R; r += 512) {
-            double b[32];
+        // rows below the diagonal block: X = P2 W^T on MFMA (K = 32)
+        const int R2 = R - 32;
+        for (int rt = wave; rt * 16 < R2; rt += 8) {
+            dx4 e0 = {0, 0, 0, 0}, e1 = e0;
+            const int r = 32 + rt * 16 + fr;
 #pragma unroll
-            for (int c = 0; c < 32; c++) b[c] = P[c * RC + r];
+            for (int s4 = 0; s4 < 8; s4++) {
+                const int kk = 4 * s4 + fk;
+                const double a = P[kk * RC + r];                 // P2(r, kk)
+                const double b0 = Ws[kk * 32 + fr];              // W(fr, kk)
+                const double b1 = Ws[kk * 32 + 16 + fr];         // W(16 + fr, kk)
+                e0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, e0, 0, 0, 0);
+                e1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, e1, 0, 0, 0);
+            }
+            // every lane's A reads of rows r are done before the stores below
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-            for (int c = 0; c < 32; c++) {
-                double s = 0.0;
-#pragma unroll
-                for (int p = 0; p < 32; p++)
-                    if (p <= c) s += b[p] * Ws[p * 32 + c];
-                if (c < jw) P[c * RC + r] = s;
+            for (int q = 0; q < 4; q++) {
+                const int rr = 32 + rt * 16 + 4 * q + fk;
+                if (rr < R) {
+                    P[fr * RC + rr] = e0[q];
+                    P[(16 + fr) * RC + rr] = e1[q];
+                }
             }
         }
         __syncthreads();
@@ -203,7 +237,7 @@ __global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int 
 //   P = C[:, j0:j0+32] - X[:, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, wave w: rows 16w..)
 //   X[:, j0:j0+32] = P W_jj^T                              (W from POTRF(k))
 constexpr int TS_ROWS = 64;
-__global__ __launch_bounds__(256) void k_trsm_tile(double *__restrict__ T, int N, int NB, int MT, int k,
+__global__ __launch_bounds__(256, 2) void k_trsm_tile(double *__restrict__ T, int N, int NB, int MT, int k,
                                                   const double *__restrict__ Winv)
 {
     __shared__ double P[32][TS_ROWS + 1];
@@ -465,17 +499,19 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double *__restrict__ T, int 
 constexpr int BM = 128, BK = 16;
 
 // workgroup L of an update launch -> tile (m, n) and its 128x128 block
-// (bm, bn); tiles enumerated column by column from column n_begin
-__device__ inline void decode_block(int L, int nsub, int n_begin, int MT, int *m, int *n, int *bm,
-                                    int *bn)
+// (bm, bn); tiles enumerated column by column from column n_begin, whose
+// first m_off tiles (rows n_begin .. n_begin+m_off-1) are skipped; the grid
+// size decides how many tiles a launch covers
+__device__ inline void decode_block(int L, int nsub, int n_begin, int m_off, int MT, int *m, int *n,
+                                    int *bm, int *bn)
 {
     const int per_tile = nsub * nsub;
     int t = L / per_tile;
     const int s = L - t * per_tile;
-    int nn = n_begin;
-    while (t >= MT - nn) { t -= MT - nn; nn++; }
+    int nn = n_begin, m0 = n_begin + m_off;
+    while (t >= MT - m0) { t -= MT - m0; nn++; m0 = nn; }
     *n = nn;
-    *m = nn + t;
+    *m = m0 + t;
     *bm = s / nsub;
     *bn = s - (*bm) * nsub;
 }
@@ -484,7 +520,7 @@ __device__ inline void decode_block(int L, int nsub, int n_begin, int MT, int *m
 // busy (one wave alone issues at half rate), so the kernel is held to 256
 // registers per lane (128 accumulators) and two workgroups share each CU.
 __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, int N, int NB, int MT, int k,
-                                                       int n_begin, int nsub)
+                                                       int n_begin, int m_off, int nsub)
 {
     __shared__ double As[2][BK][BM];
     __shared__ double Bs[2][BK][BM];
@@ -499,7 +535,7 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, 
     const int L = (bid < G8) ? (bid % 8) * (G8 / 8) + bid / 8 : bid;
 
     int m, n, bm, bn;
-    decode_block(L, nsub, n_begin, MT, &m, &n, &bm, &bn);
+    decode_block(L, nsub, n_begin, m_off, MT, &m, &n, &bm, &bn);
     if (m == n && bm < bn) return;                 // strictly upper blocks of a diagonal tile
 
     const int rows_m = tile_rows(N, NB, MT, m);
@@ -614,7 +650,7 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, 
 // and four B pairs (4 distinct per wave: broadcast) for 64 FMAs.
 
 __global__ __launch_bounds__(256) void k_update_valu(double *__restrict__ T, int N, int NB, int MT, int k,
-                                                    int n_begin, int nsub)
+                                                    int n_begin, int m_off, int nsub)
 {
     __shared__ double As[2][BK][BM];
     __shared__ double Bs[2][BK][BM];
@@ -624,7 +660,7 @@ __global__ __launch_bounds__(256) void k_update_valu(double *__restrict__ T, int
     const int G8 = (G / 8) * 8;
     const int L = (bid < G8) ? (bid % 8) * (G8 / 8) + bid / 8 : bid;
     int m, n, bm, bn;
-    decode_block(L, nsub, n_begin, MT, &m, &n, &bm, &bn);
+    decode_block(L, nsub, n_begin, m_off, MT, &m, &n, &bm, &bn);
     if (m == n && bm < bn) return;
     const int rows_m = tile_rows(N, NB, MT, m);
     const int cols_n = tile_rows(N, NB, MT, n);
@@ -754,32 +790,11 @@ int g_panel_kernel = 1;
 int potrf_lds_bytes(int NB, int *RC)
 {
     if (g_panel_kernel == 1) {
-        *RC = (NB + 15) & ~15;
-        return (32 * (*RC) + 1024 + 32) * (int)sizeof(double);
+        *RC = ((NB + 15) & ~15) + 1;
+        return (32 * (*RC) + 1024) * (int)sizeof(double);
     }
     *RC = ((NB + 3) & ~3) + 4;
     return (PJB * (*RC) + 1024) * (int)sizeof(double);
-}
-
-double update_flops(int N, int NB, int MT, int n_begin, int n_end)
-{
-    double f = 0.0;
-    for (int n = n_begin; n < n_end; n++)
-        for (int m = n; m < MT; m++) {
-            double tm = tile_rows(N, NB, MT, m);
-            if (m == n) f += (double)NB * tm * (tm + 1.0);
-            else f += 2.0 * tm * NB * NB;
-        }
-    return f;
-}
-
-double update_bytes(int N, int NB, int MT, int n_begin, int n_end)
-{
-    // C read + written once, each panel tile of column k read once
-    double b = 0.0;
-    for (int n = n_begin; n < n_end; n++)
-        for (int m = n; m < MT; m++) b += 2.0 * tile_rows(N, NB, MT, m) * NB * 8.0;
-    return b + (double)(MT - n_begin) * NB * NB * 8.0;
 }
 
 int launch_potrf(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, double *Winv)
@@ -789,10 +804,10 @@ int launch_potrf(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k
     const int n = tile_rows(N, NB, MT, k);
     double *A = dT + tile_index(MT, k, k) * (int64_t)NB * NB;
     const double fl = (double)n * n * n / 3.0;
-    if (g_panel_kernel == 1)
+    if (g_panel_kernel == 1) {
         MI_LAUNCH(c, s, "dpotrf_potrf_diag", fl, 2.0 * n * n * 8.0,
                   k_potrf_tile<<<1, 512, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
-    else
+    } else
         MI_LAUNCH(c, s, "dpotrf_potrf_diag", fl, 2.0 * n * n * 8.0,
                   k_potrf_diag<<<1, POTRF_THREADS, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
     return 0;
@@ -819,29 +834,49 @@ int launch_trsm(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k,
 // trailing-update engine: 0 = FP64 VALU, 1 = FP64 MFMA (default)
 int g_update_kernel = -1;
 
-int launch_update(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, int n_begin,
-                  int n_end)
+// Trailing update of step k over a run of tiles: column n_begin from row
+// n_begin + m_off down, then whole columns, ntiles tiles in all (ntiles < 0:
+// through the last column).
+int launch_update_run(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, int n_begin,
+                      int m_off, int64_t ntiles)
 {
-    if (n_begin >= n_end) return 0;
+    if (n_begin >= MT || n_begin + m_off >= MT) return 0;
+    int64_t avail = MT - n_begin - m_off;
+    for (int n = n_begin + 1; n < MT; n++) avail += MT - n;
+    if (ntiles < 0 || ntiles > avail) ntiles = avail;
+    if (ntiles <= 0) return 0;
     const int nsub = (NB + BM - 1) / BM;
-    int64_t ntiles = 0;
-    for (int n = n_begin; n < n_end; n++) ntiles += MT - n;
     const int64_t grid = ntiles * nsub * nsub;
-    if (grid <= 0) return 0;
     if (grid > 0x7fffffff) { set_error("update grid too large"); return -3; }
+    // algorithmic flops / bytes of the covered tiles
+    double fl = 0.0, by = 0.0;
+    int64_t left = ntiles;
+    for (int n = n_begin; n < MT && left > 0; n++)
+        for (int m = (n == n_begin ? n + m_off : n); m < MT && left > 0; m++, left--) {
+            const double tm = tile_rows(N, NB, MT, m);
+            fl += (m == n) ? (double)NB * tm * (tm + 1.0) : 2.0 * tm * NB * NB;
+            by += 2.0 * tm * NB * 8.0 + NB * NB * 8.0;
+        }
     if (g_update_kernel < 0) {
         const char *e = getenv("DPLASMA_MI_UPDATE");
         g_update_kernel = (e && !strcmp(e, "valu")) ? 0 : 1;
     }
     if (g_update_kernel == 1)
-        MI_LAUNCH(c, s, "dpotrf_update", update_flops(N, NB, MT, n_begin, n_end),
-                  update_bytes(N, NB, MT, n_begin, n_end),
-                  k_update_mfma<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, nsub));
+        MI_LAUNCH(c, s, "dpotrf_update", fl, by,
+                  k_update_mfma<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
     else
-        MI_LAUNCH(c, s, "dpotrf_update_valu", update_flops(N, NB, MT, n_begin, n_end),
-                  update_bytes(N, NB, MT, n_begin, n_end),
-                  k_update_valu<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, nsub));
+        MI_LAUNCH(c, s, "dpotrf_update_valu", fl, by,
+                  k_update_valu<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
     return 0;
+}
+
+int launch_update(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, int n_begin,
+                  int n_end)
+{
+    if (n_begin >= n_end) return 0;
+    int64_t ntiles = 0;
+    for (int n = n_begin; n < n_end; n++) ntiles += MT - n;
+    return launch_update_run(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles);
 }
 
 
@@ -959,13 +994,18 @@ int mi_dpotrf_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
 
     for (int k = 0; k < MT - 1; k++) {
         MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
-        // column k+1 first: it gates the next panel
-        if ((rc = launch_update(c, c->s_main, dT, N, NB, MT, k, k + 1, k + 2))) return rc;
+        // the next diagonal tile first: POTRF(k+1) only waits for its SYRK
+        if ((rc = launch_update_run(c, c->s_main, dT, N, NB, MT, k, k + 1, 0, 1))) return rc;
+        hipEvent_t dk = next_event(c);
+        MI_CHECK(hipEventRecord(dk, c->s_main));
+        // then the rest of column k+1, which gates TRSM(k+1)
+        if ((rc = launch_update_run(c, c->s_main, dT, N, NB, MT, k, k + 1, 1, MT - k - 2))) return rc;
         hipEvent_t ck = next_event(c);
         MI_CHECK(hipEventRecord(ck, c->s_main));
-        MI_CHECK(hipStreamWaitEvent(c->s_panel, ck, 0));
+        MI_CHECK(hipStreamWaitEvent(c->s_panel, dk, 0));
         double *w = g_ws.winv + ((k + 1) & 1) * wsz;
         if ((rc = launch_potrf(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
+        MI_CHECK(hipStreamWaitEvent(c->s_panel, ck, 0));
         if ((rc = launch_trsm(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
         pk = next_event(c);
         MI_CHECK(hipEventRecord(pk, c->s_panel));
