@@ -112,6 +112,18 @@ def libs():
     _sig(M, "mi_dpotrf_tiles", i, [vp, i, i, vp, pi])
     _sig(M, "mi_dtrsm_LL", i, [vp, i, i, d, i, i, vp, i, vp, i])
     _sig(M, "mi_dpotrs_tiles", i, [vp, i, i, vp, i, vp, i])
+    _sig(M, "mi_spotrf_tiles", i, [vp, i, i, vp, pi])
+    _sig(M, "mi_spotrf_step_potrf", i, [vp, i, i, vp, i, pi])
+    _sig(M, "mi_dsposv_tiles", i, [vp, i, i, vp, i, vp, i, vp, i, i, pi, pi])
+    _sig(M, "mi_dgauss", i, [vp, ctypes.c_int64, vp, u64])
+    _sig(M, "mi_dorth_haar", i, [vp, i, u64, vp])
+    _sig(M, "mi_dspd_from_orth_tiles", i, [vp, i, i, vp, d, vp])
+    _sig(M, "mi_dsymm_resid", i, [vp, i, i, vp, i, vp, i, vp, i, vp, i])
+    _sig(M, "mi_dlag2s_tiles", i, [vp, ctypes.c_int64, vp, vp, pi])
+    _sig(M, "mi_spotrf_step_trsm", i, [vp, i, i, vp, i])
+    _sig(M, "mi_spotrf_step_update", i, [vp, i, i, vp, i, i, i])
+    _sig(M, "mi_strsm_LL", i, [vp, i, i, ctypes.c_float, i, i, vp, i, vp, i])
+    _sig(M, "mi_spotrs_tiles", i, [vp, i, i, vp, i, vp, i])
     _sig(M, "mi_dpotrf_step_potrf", i, [vp, i, i, vp, i, pi])
     _sig(M, "mi_dpotrf_step_trsm", i, [vp, i, i, vp, i])
     _sig(M, "mi_dpotrf_step_update", i, [vp, i, i, vp, i, i, i])

@@ -59,6 +59,15 @@ hipEvent_t next_event(Ctx *c);
 void prof_begin(Ctx *c, hipStream_t s, const char *name, double flops, double bytes, ProfRec *r);
 void prof_end(Ctx *c, hipStream_t s, ProfRec *r);
 
+// Tile triangular solves (mi_solve.hip), enqueue only: trsm_prepare writes
+// the 32x32 diagonal-block inverses of L (trsm_ws_elems(N, NB) elements),
+// trsm_enqueue applies op(L)^{-1} (trans 111 / 112) to NRHS columns.
+size_t trsm_ws_elems(int N, int NB);
+template <class T> int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int NB, const T *dT, T *W);
+template <class T>
+int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, const T *W, int NRHS, T *dB,
+                 int ldb);
+
 }  // namespace mi
 
 // the opaque handle of the C ABI is the context itself

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include "mi_common.hpp"
 #include "dplasma_mi.h"
 
@@ -36,6 +37,32 @@ namespace {
 constexpr int PJB = 32;            // panel width inside the diagonal factorization
 constexpr int POTRF_THREADS = 512;
 typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
+
+// Element-type traits of the 16x16x4 MFMA shapes used here: FP64
+// (v_mfma_f64_16x16x4_f64) and FP32 (v_mfma_f32_16x16x4_f32).  Both take one
+// A and one B element per lane (A(i, k) and B(k, j) in lane 16k + i, resp.
+// 16k + j) and return four elements per lane; row(q, l/16) is the output row
+// of element q (column l % 16), as probed on gfx950.
+template <class T> struct MF;
+template <> struct MF<double> {
+    typedef dx4 v4;
+    typedef double2 v2;
+    __device__ static inline v4 mfma(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int q, int fk) { return 4 * q + fk; }
+};
+template <> struct MF<float> {
+    typedef fx4 v4;
+    typedef float2 v2;
+    __device__ static inline v4 mfma(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int q, int fk) { return 4 * fk + q; }
+};
 
 // ---------------------------------------------------------------------------
 // One wave: acc[ct] += X[r0+i][0:K] * Y[c0+16ct+j][0:K]^T for the 16x16 output
@@ -44,37 +71,39 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 // beyond xr_lim / yr_lim read as zero.  K is a multiple of 32.  Operand lane
 // layout: A(i, k) and B(k, j) sit in lane 16k + i (resp. 16k + j); result
 // element q of lane l is D(4q + l/16, l%16).
-__device__ inline void mfma_panel_2(const double *__restrict__ X, int ldx, int r0, int xr_lim,
-                                    const double *__restrict__ Y, int ldy, int c0, int yr_lim, int K,
-                                    dx4 acc[2])
+template <class T>
+__device__ inline void mfma_panel_2(const T *__restrict__ X, int ldx, int r0, int xr_lim,
+                                    const T *__restrict__ Y, int ldy, int c0, int yr_lim, int K,
+                                    typename MF<T>::v4 acc[2])
 {
+    typedef typename MF<T>::v4 V4;
     const int lane = threadIdx.x & 63, fr = lane & 15, fk = lane >> 4;
     const int xr = r0 + fr, y0 = c0 + fr, y1 = c0 + 16 + fr;
     const bool xv = xr < xr_lim, yv0 = y0 < yr_lim, yv1 = y1 < yr_lim;
-    dx4 e0 = {0, 0, 0, 0}, e1 = e0, o0 = e0, o1 = e0;
+    V4 e0 = {0, 0, 0, 0}, e1 = e0, o0 = e0, o1 = e0;
     // 32 columns of K per pass; the operands of pass p+1 are in flight while
     // the 16 MFMAs of pass p run
-    double a[8], b0[8], b1[8];
-    auto load = [&](int k, double *pa, double *pb0, double *pb1) {
+    T a[8], b0[8], b1[8];
+    auto load = [&](int k, T *pa, T *pb0, T *pb1) {
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int64_t kk = k + 4 * s + fk;
-            pa[s] = xv ? X[kk * ldx + xr] : 0.0;
-            pb0[s] = yv0 ? Y[kk * ldy + y0] : 0.0;
-            pb1[s] = yv1 ? Y[kk * ldy + y1] : 0.0;
+            pa[s] = xv ? X[kk * ldx + xr] : T(0);
+            pb0[s] = yv0 ? Y[kk * ldy + y0] : T(0);
+            pb1[s] = yv1 ? Y[kk * ldy + y1] : T(0);
         }
     };
     if (K > 0) load(0, a, b0, b1);
     for (int k = 0; k < K; k += 32) {
-        double na[8], nb0[8], nb1[8];
+        T na[8], nb0[8], nb1[8];
         const bool more = k + 32 < K;
         if (more) load(k + 32, na, nb0, nb1);
 #pragma unroll
         for (int s = 0; s < 8; s += 2) {
-            e0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b0[s], e0, 0, 0, 0);
-            e1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b1[s], e1, 0, 0, 0);
-            o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s + 1], b0[s + 1], o0, 0, 0, 0);
-            o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s + 1], b1[s + 1], o1, 0, 0, 0);
+            e0 = MF<T>::mfma(a[s], b0[s], e0);
+            e1 = MF<T>::mfma(a[s], b1[s], e1);
+            o0 = MF<T>::mfma(a[s + 1], b0[s + 1], o0);
+            o1 = MF<T>::mfma(a[s + 1], b1[s + 1], o1);
         }
         if (more) {
 #pragma unroll
@@ -83,6 +112,45 @@ __device__ inline void mfma_panel_2(const double *__restrict__ X, int ldx, int r
     }
     acc[0] += e0 + o0;
     acc[1] += e1 + o1;
+}
+
+// One wave, up to four 16-row tiles at once: acc[t][ct] += X[r0+128t+i][0:K] *
+// Y[16ct+j][0:K]^T for t < nt (row tiles r0, r0 + 128, r0 + 256, r0 + 384),
+// ct = 0, 1; X and Y share the leading dimension ld (K * ld < 2^31).  All operands of a 32-wide K pass (4 x 8 A values, 2 x 8 B
+// values per lane) are in flight together, so the global-load latency is
+// paid once per pass for the four tiles.  Layout and masking as mfma_panel_2.
+template <class T>
+__device__ inline void mfma_panel_4x2(const T *__restrict__ X, int ld, int r0, int nt, int xr_lim,
+                                      const T *__restrict__ Y, int yr_lim, int K,
+                                      typename MF<T>::v4 acc[4][2])
+{
+    const int lane = threadIdx.x & 63, fr = lane & 15, fk = lane >> 4;
+    const bool yv0 = fr < yr_lim, yv1 = 16 + fr < yr_lim;
+    // wave-uniform row-tile bases, one 32-bit per-lane offset per K slice
+    bool xv[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) xv[t] = t < nt && r0 + 128 * t + fr < xr_lim;
+    const T *Xt0 = X + r0, *Xt1 = X + r0 + 128, *Xt2 = X + r0 + 256, *Xt3 = X + r0 + 384;
+    for (int k = 0; k < K; k += 32) {
+        T a[4][8], b0[8], b1[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int off = (k + 4 * s + fk) * ld + fr;
+            b0[s] = yv0 ? Y[off] : T(0);
+            b1[s] = yv1 ? Y[off + 16] : T(0);
+            a[0][s] = xv[0] ? Xt0[off] : T(0);
+            a[1][s] = xv[1] ? Xt1[off] : T(0);
+            a[2][s] = xv[2] ? Xt2[off] : T(0);
+            a[3][s] = xv[3] ? Xt3[off] : T(0);
+        }
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                acc[t][0] = MF<T>::mfma(a[t][s], b0[s], acc[t][0]);
+                acc[t][1] = MF<T>::mfma(a[t][s], b1[s], acc[t][1]);
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -94,24 +162,36 @@ __device__ inline double readlane_d(double v, int lane)
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+__device__ inline double readlane_t(double v, int lane) { return readlane_d(v, lane); }
+__device__ inline float readlane_t(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 
 // POTRF(k), left-looking by 32-column panels in one workgroup of 8 waves:
-//   P = A[j0:n, j0:j0+32] - L[j0:n, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, LDS P)
-//   L_jj = chol(P[0:32]), W_jj = L_jj^{-1}   wave 0, one row (resp. one
-//                                            column of W) per lane, values
-//                                            broadcast with v_readlane: no
-//                                            LDS round trip on the serial path
-//   P[32:] <- P[32:] W_jj^T                  MFMA, K = 32
+//   P = A[j0:n, j0:j0+32] - L[j0:n, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, LDS P;
+//       wave w owns the 16-row tiles w, w+8, w+16, w+24 and runs them together)
+//   wave 0, one sweep over the 32 columns c:
+//       lanes 0..31:  row r of L_jj = chol(P[0:32])   (right-looking, in registers)
+//       lanes 32..63: column j of W_jj = L_jj^{-1}    (forward substitution)
+//     both updates are the same FMA  v[c2] -= m * L(c2, c)  with the
+//     multiplier m = L(r, c) (resp. W(c, j)) and L(c2, c) broadcast by
+//     v_readlane, so the serial path has no LDS round trip and no branch;
+//     columns past the tile edge get an identity diagonal
+//   P[32:] <- P[32:] W_jj^T                  (MFMA, K = 32)
 // then the panel is written back.  W_jj goes to Winv for the panel TRSM.
 // info = k*NB + j of the first non-positive pivot j (1-based), as
 // CORE_dpotrf reports it (src/zpotrf_L.jdf:200-202).
-__global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int n, int ld, int RC,
+template <class T>
+__global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, int ld, int RC,
                                                    int info_base, int *__restrict__ d_info,
-                                                   double *__restrict__ Winv)
+                                                   T *__restrict__ Winv)
 {
-    extern __shared__ double smem[];
-    double *P = smem;               // 32 columns x RC rows (RC odd: no bank aliasing)
-    double *Ws = P + 32 * RC;       // W(i, j) at Ws[j*32 + i]
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    typedef typename MF<T>::v4 V4;
+    T *smem = reinterpret_cast<T *>(smem_raw);
+    T *P = smem;                    // 32 columns x RC rows (RC odd: no bank aliasing)
+    T *Ws = P + 32 * RC;            // W(i, j) at Ws[j*32 + i]
     __shared__ int s_fail;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
@@ -120,75 +200,60 @@ __global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int 
 
     for (int j0 = 0; j0 < n; j0 += 32) {
         const int jw = min(32, n - j0), R = n - j0, R16 = (R + 15) >> 4;
-        double *Aj = A + j0;                     // row j0 of column 0
-        for (int rt = wave; rt < R16; rt += 8) {
-            dx4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-            if (j0 > 0) mfma_panel_2(Aj, ld, rt * 16, R, Aj, ld, 0, jw, j0, acc);
+        T *Aj = A + j0;                          // row j0 of column 0
+        for (int g0 = wave; g0 < R16; g0 += 32) {
+            const int nt = min(4, (R16 - g0 + 7) >> 3);     // row tiles g0 + 8t < R16
+            V4 acc[4][2];
 #pragma unroll
-            for (int ct = 0; ct < 2; ct++)
+            for (int t = 0; t < 4; t++) acc[t][0] = acc[t][1] = V4{T(0), T(0), T(0), T(0)};
+            if (j0 > 0) mfma_panel_4x2<T>(Aj, ld, g0 * 16, nt, R, Aj, jw, j0, acc);
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int r = rt * 16 + 4 * q + fk, c = ct * 16 + fr;
-                    P[c * RC + r] = (r < R && c < jw) ? Aj[(int64_t)(j0 + c) * ld + r] - acc[ct][q] : 0.0;
-                }
+            for (int t = 0; t < 4; t++) {
+                if (t >= nt) break;
+                const int rt = g0 + 8 * t;
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int r = rt * 16 + MF<T>::row(q, fk), c = ct * 16 + fr;
+                        P[c * RC + r] = (r < R && c < jw) ? Aj[(int64_t)(j0 + c) * ld + r] - acc[t][ct][q] : T(0);
+                    }
+            }
         }
         __syncthreads();
 
         if (wave == 0) {
-            double row[32];
-            bool failed = false;
+            // v[c]: lanes < 32 hold P(lane, c), lanes >= 32 hold e_{lane-32}(c)
+            T v[32];
+            const bool lrow = lane < 32;
+            int j = lane - 32;
+            asm volatile("" : "+v"(j));             // built per panel: keeps the identity out of LICM
 #pragma unroll
-            for (int c = 0; c < 32; c++) row[c] = (lane < 32) ? P[c * RC + lane] : 0.0;
+            for (int c = 0; c < 32; c++) v[c] = lrow ? P[c * RC + lane] : (c == j ? T(1) : T(0));
+            unsigned bad = 0;                       // non-positive pivots, bit c
 #pragma unroll
             for (int c = 0; c < 32; c++) {
-                if (c < jw && !failed) {
-                    const double d0 = readlane_d(row[c], c);
-                    if (!(d0 > 0.0)) {
-                        if (lane == 0) s_fail = j0 + c + 1;
-                        failed = true;       // uniform: d0 is wave-uniform
-                        continue;
-                    }
-                    const double d = sqrt(d0);
-                    const double l = row[c] * (1.0 / d);
-                    row[c] = (lane == c) ? d : ((lane > c) ? l : row[c]);
+                T d0 = readlane_t(v[c], c);
+                const bool ok = c < jw ? d0 > T(0) : true;
+                bad |= (ok ? 0u : 1u) << c;
+                d0 = ok && c < jw ? d0 : T(1);
+                const T d = sqrt(d0);
+                const T rd = T(1) / d;
+                const T sc = v[c] * rd;             // L(lane, c) below the diagonal / W(c, j)
+                const T m = (!lrow || lane > c) ? sc : T(0);
+                v[c] = lrow ? (lane == c ? d : (lane > c ? sc : v[c])) : sc;
 #pragma unroll
-                    for (int c2 = c + 1; c2 < 32; c2++) {
-                        const double lc2 = readlane_d(l, c2);    // L(c2, c)
-                        if (lane > c) row[c2] -= l * lc2;
-                    }
-                }
-                __builtin_amdgcn_sched_barrier(0);
+                for (int c2 = c + 1; c2 < 32; c2++) v[c2] -= m * readlane_t(m, c2);
             }
-            if (lane < 32) {
+            if (lrow) {
 #pragma unroll
                 for (int c = 0; c < 32; c++)
-                    if (c <= lane) P[c * RC + lane] = row[c];
+                    if (c <= lane) P[c * RC + lane] = v[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 32; c++) Ws[j * 32 + c] = v[c];
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (!failed) {
-                // W = L^{-1}: lane j builds column j by the column sweep,
-                // column c of L read from LDS as one batch per step
-                double w[32];
-#pragma unroll
-                for (int i = 0; i < 32; i++) w[i] = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-                for (int c = 0; c < 32; c++) {
-                    if (c < jw) {
-                        const double *Lc = P + c * RC;
-                        w[c] *= 1.0 / Lc[c];
-#pragma unroll
-                        for (int r = c + 1; r < 32; r++)
-                            if (r < jw) w[r] -= Lc[r] * w[c];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);   // keep each step's loads in its step
-                }
-                if (lane < 32) {
-#pragma unroll
-                    for (int c = 0; c < 32; c++) Ws[lane * 32 + c] = w[c];
-                }
-            }
+            if (bad && lane == 0) s_fail = j0 + __builtin_ctz(bad) + 1;
         }
         __syncthreads();
         if (s_fail) break;
@@ -197,16 +262,16 @@ __global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int 
         // rows below the diagonal block: X = P2 W^T on MFMA (K = 32)
         const int R2 = R - 32;
         for (int rt = wave; rt * 16 < R2; rt += 8) {
-            dx4 e0 = {0, 0, 0, 0}, e1 = e0;
+            V4 e0 = {T(0), T(0), T(0), T(0)}, e1 = e0;
             const int r = 32 + rt * 16 + fr;
 #pragma unroll
             for (int s4 = 0; s4 < 8; s4++) {
                 const int kk = 4 * s4 + fk;
-                const double a = P[kk * RC + r];                 // P2(r, kk)
-                const double b0 = Ws[kk * 32 + fr];              // W(fr, kk)
-                const double b1 = Ws[kk * 32 + 16 + fr];         // W(16 + fr, kk)
-                e0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, e0, 0, 0, 0);
-                e1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, e1, 0, 0, 0);
+                const T a = P[kk * RC + r];                 // P2(r, kk)
+                const T b0 = Ws[kk * 32 + fr];              // W(fr, kk)
+                const T b1 = Ws[kk * 32 + 16 + fr];         // W(16 + fr, kk)
+                e0 = MF<T>::mfma(a, b0, e0);
+                e1 = MF<T>::mfma(a, b1, e1);
             }
             // every lane's A reads of rows r are done before the stores below
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -214,7 +279,7 @@ __global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int 
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int rr = 32 + rt * 16 + 4 * q + fk;
+                const int rr = 32 + rt * 16 + MF<T>::row(q, fk);
                 if (rr < R) {
                     P[fr * RC + rr] = e0[q];
                     P[(16 + fr) * RC + rr] = e1[q];
@@ -237,11 +302,13 @@ __global__ __launch_bounds__(512) void k_potrf_tile(double *__restrict__ A, int 
 //   P = C[:, j0:j0+32] - X[:, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, wave w: rows 16w..)
 //   X[:, j0:j0+32] = P W_jj^T                              (W from POTRF(k))
 constexpr int TS_ROWS = 64;
-__global__ __launch_bounds__(256, 2) void k_trsm_tile(double *__restrict__ T, int N, int NB, int MT, int k,
-                                                  const double *__restrict__ Winv)
+template <class T>
+__global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N, int NB, int MT, int k,
+                                                  const T *__restrict__ Winv)
 {
-    __shared__ double P[32][TS_ROWS + 1];
-    __shared__ double Ws[1024];
+    __shared__ T P[32][TS_ROWS + 1];
+    __shared__ T Ws[1024];
+    typedef typename MF<T>::v4 V4;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
     const int strips = (NB + TS_ROWS - 1) / TS_ROWS;
@@ -250,27 +317,27 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(double *__restrict__ T, in
     const int rows_m = tile_rows(N, NB, MT, m);
     if (r0 >= rows_m) return;
     const int rv = min(TS_ROWS, rows_m - r0);
-    double *C = T + tile_index(MT, m, k) * (int64_t)NB * NB + r0;
-    const double *L = T + tile_index(MT, k, k) * (int64_t)NB * NB;
+    T *C = Tt + tile_index(MT, m, k) * (int64_t)NB * NB + r0;
+    const T *L = Tt + tile_index(MT, k, k) * (int64_t)NB * NB;
     const int r = tid & 63, cg = tid >> 6;
 
     for (int j0 = 0; j0 < NB; j0 += 32) {
         const int jw = min(32, NB - j0);
-        dx4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        V4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
         if (j0 > 0) mfma_panel_2(C, NB, wave * 16, rv, L + j0, NB, 0, jw, j0, acc);
 #pragma unroll
         for (int ct = 0; ct < 2; ct++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int rr = wave * 16 + 4 * q + fk, c = ct * 16 + fr;
-                P[c][rr] = (rr < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + rr] - acc[ct][q] : 0.0;
+                const int rr = wave * 16 + MF<T>::row(q, fk), c = ct * 16 + fr;
+                P[c][rr] = (rr < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + rr] - acc[ct][q] : T(0);
             }
         for (int idx = tid; idx < 1024; idx += 256) Ws[idx] = Winv[(j0 >> 5) * 1024 + idx];
         __syncthreads();
 #pragma unroll
         for (int jj = 0; jj < 8; jj++) {
             const int c = cg * 8 + jj;
-            double s = 0.0;
+            T s = T(0);
             for (int p = 0; p <= c; p++) s += P[p][r] * Ws[p * 32 + c];
             if (r < rv && c < jw) C[(int64_t)(j0 + c) * NB + r] = s;
         }
@@ -519,11 +586,14 @@ __device__ inline void decode_block(int L, int nsub, int n_begin, int m_off, int
 // The FP64 MFMA pipe of a gfx950 SIMD needs two resident waves to stay
 // busy (one wave alone issues at half rate), so the kernel is held to 256
 // registers per lane (128 accumulators) and two workgroups share each CU.
-__global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, int N, int NB, int MT, int k,
+template <class T>
+__global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int N, int NB, int MT, int k,
                                                        int n_begin, int m_off, int nsub)
 {
-    __shared__ double As[2][BK][BM];
-    __shared__ double Bs[2][BK][BM];
+    __shared__ T As[2][BK][BM];
+    __shared__ T Bs[2][BK][BM];
+    typedef typename MF<T>::v4 V4;
+    typedef typename MF<T>::v2 V2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
 
@@ -544,27 +614,27 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, 
     if (r_base >= rows_m || c_base >= cols_n) return;
     const bool full = (r_base + BM <= rows_m) && (c_base + BM <= cols_n) && (NB % BK == 0) && !(NB & 1);
 
-    const double *Ap = T + tile_index(MT, m, k) * (int64_t)NB * NB;
-    const double *Bp = T + tile_index(MT, n, k) * (int64_t)NB * NB;
-    double *Cp = T + tile_index(MT, m, n) * (int64_t)NB * NB;
+    const T *Ap = Tt + tile_index(MT, m, k) * (int64_t)NB * NB;
+    const T *Bp = Tt + tile_index(MT, n, k) * (int64_t)NB * NB;
+    T *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
 
-    dx4 acc[4][4];
+    V4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 4; j++) acc[i][j] = V4{T(0), T(0), T(0), T(0)};
 
     // global -> register staging in 16-byte pairs: pair e = tid + 256*i,
     // k row kk = e / 64, rows 2*(e % 64) and 2*(e % 64) + 1
-    double2 ra[4], rb[4];
+    V2 ra[4], rb[4];
     auto gload = [&](int k0) {
         if (full) {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int e = tid + 256 * i;
                 const int kk = e >> 6, r = (e & 63) * 2;
-                ra[i] = *(const double2 *)(Ap + (int64_t)(k0 + kk) * NB + r_base + r);
-                rb[i] = *(const double2 *)(Bp + (int64_t)(k0 + kk) * NB + c_base + r);
+                ra[i] = *(const V2 *)(Ap + (int64_t)(k0 + kk) * NB + r_base + r);
+                rb[i] = *(const V2 *)(Bp + (int64_t)(k0 + kk) * NB + c_base + r);
             }
         } else {
 #pragma unroll
@@ -572,17 +642,17 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, 
                 const int e = tid + 256 * i;
                 const int kk = e >> 6, r = (e & 63) * 2;
                 const int kg = k0 + kk;
-                double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+                T a0 = T(0), a1 = T(0), b0 = T(0), b1 = T(0);
                 if (kg < NB) {
-                    const double *ac = Ap + (int64_t)kg * NB;
-                    const double *bc = Bp + (int64_t)kg * NB;
+                    const T *ac = Ap + (int64_t)kg * NB;
+                    const T *bc = Bp + (int64_t)kg * NB;
                     if (r_base + r < rows_m) a0 = ac[r_base + r];
                     if (r_base + r + 1 < rows_m) a1 = ac[r_base + r + 1];
                     if (c_base + r < cols_n) b0 = bc[c_base + r];
                     if (c_base + r + 1 < cols_n) b1 = bc[c_base + r + 1];
                 }
-                ra[i] = make_double2(a0, a1);
-                rb[i] = make_double2(b0, b1);
+                ra[i] = V2{a0, a1};
+                rb[i] = V2{b0, b1};
             }
         }
     };
@@ -591,8 +661,8 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, 
         for (int i = 0; i < 4; i++) {
             const int e = tid + 256 * i;
             const int kk = e >> 6, r = (e & 63) * 2;
-            *(double2 *)&As[buf][kk][r] = ra[i];
-            *(double2 *)&Bs[buf][kk][r] = rb[i];
+            *(V2 *)&As[buf][kk][r] = ra[i];
+            *(V2 *)&Bs[buf][kk][r] = rb[i];
         }
     };
 
@@ -606,7 +676,7 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, 
         if (st + 1 < nstage) gload((st + 1) * BK);
 #pragma unroll
         for (int kq = 0; kq < BK / 4; kq++) {
-            double a[4], b[4];
+            T a[4], b[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) a[i] = As[buf][kq * 4 + fk][wm * 64 + i * 16 + fr];
 #pragma unroll
@@ -615,7 +685,7 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, 
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = MF<T>::mfma(a[i], b[j], acc[i][j]);
         }
         if (st + 1 < nstage) lstore(buf ^ 1);
         __syncthreads();
@@ -632,7 +702,7 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(double *__restrict__ T, 
             if (col >= cols_n) continue;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int rr = r_base + wm * 64 + i * 16 + 4 * q + fk;
+                const int rr = r_base + wm * 64 + i * 16 + MF<T>::row(q, fk);
                 if (rr < rows_m && (!diag || rr >= col))
                     Cp[(int64_t)col * NB + rr] -= acc[i][j][q];
             }
@@ -779,65 +849,81 @@ __global__ void k_fill_int(int *p, int v, int n)
 }
 
 // ---------------------------------------------------------------------------
-struct Plan {
-    int N, NB, MT, RC;
-    size_t potrf_lds;
+// panel engine: 0 = VALU diagonal factor / solve (FP64 only), 1 = MFMA panel updates (default)
+int g_panel_kernel = 1;
+// trailing-update engine: 0 = FP64 VALU, 1 = MFMA (default; FP32 always MFMA)
+int g_update_kernel = -1;
+
+// profiler names per precision
+template <class T> struct KN;
+template <> struct KN<double> {
+    static constexpr const char *potrf = "dpotrf_potrf_diag";
+    static constexpr const char *trsm = "dpotrf_trsm_panel";
+    static constexpr const char *update = "dpotrf_update";
+};
+template <> struct KN<float> {
+    static constexpr const char *potrf = "spotrf_potrf_diag";
+    static constexpr const char *trsm = "spotrf_trsm_panel";
+    static constexpr const char *update = "spotrf_update";
 };
 
-// panel engine: 0 = VALU diagonal factor / solve, 1 = MFMA panel updates (default)
-int g_panel_kernel = 1;
+template <class T> bool valu_panel() { return std::is_same<T, double>::value && g_panel_kernel == 0; }
 
+template <class T>
 int potrf_lds_bytes(int NB, int *RC)
 {
-    if (g_panel_kernel == 1) {
+    if (!valu_panel<T>()) {
         *RC = ((NB + 15) & ~15) + 1;
-        return (32 * (*RC) + 1024) * (int)sizeof(double);
+        return (32 * (*RC) + 1024) * (int)sizeof(T);
     }
     *RC = ((NB + 3) & ~3) + 4;
     return (PJB * (*RC) + 1024) * (int)sizeof(double);
 }
 
-int launch_potrf(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, double *Winv)
+template <class T>
+int launch_potrf(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, T *Winv)
 {
     int RC;
-    int lds = potrf_lds_bytes(NB, &RC);
+    int lds = potrf_lds_bytes<T>(NB, &RC);
     const int n = tile_rows(N, NB, MT, k);
-    double *A = dT + tile_index(MT, k, k) * (int64_t)NB * NB;
+    T *A = dT + tile_index(MT, k, k) * (int64_t)NB * NB;
     const double fl = (double)n * n * n / 3.0;
-    if (g_panel_kernel == 1) {
-        MI_LAUNCH(c, s, "dpotrf_potrf_diag", fl, 2.0 * n * n * 8.0,
-                  k_potrf_tile<<<1, 512, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
-    } else
-        MI_LAUNCH(c, s, "dpotrf_potrf_diag", fl, 2.0 * n * n * 8.0,
-                  k_potrf_diag<<<1, POTRF_THREADS, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
+    if (!valu_panel<T>()) {
+        MI_LAUNCH(c, s, KN<T>::potrf, fl, 2.0 * n * n * sizeof(T),
+                  k_potrf_tile<T><<<1, 512, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
+    } else {
+        MI_LAUNCH(c, s, KN<T>::potrf, fl, 2.0 * n * n * sizeof(T),
+                  k_potrf_diag<<<1, POTRF_THREADS, lds, s>>>((double *)A, n, NB, RC, k * NB, c->d_info,
+                                                              (double *)Winv));
+    }
     return 0;
 }
 
-int launch_trsm(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, const double *Winv)
+template <class T>
+int launch_trsm(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, const T *Winv)
 {
     const int ntile = MT - k - 1;
     if (ntile <= 0) return 0;
     double fl = 0.0;
     for (int m = k + 1; m < MT; m++) fl += (double)tile_rows(N, NB, MT, m) * NB * NB;
-    if (g_panel_kernel == 1) {
+    if (!valu_panel<T>()) {
         const int strips = (NB + TS_ROWS - 1) / TS_ROWS;
-        MI_LAUNCH(c, s, "dpotrf_trsm_panel", fl, fl / NB * 16.0,
-                  k_trsm_tile<<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+        MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
+                  k_trsm_tile<T><<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
     } else {
         const int strips = (NB + TRSM_ROWS - 1) / TRSM_ROWS;
-        MI_LAUNCH(c, s, "dpotrf_trsm_panel", fl, fl / NB * 16.0,
-                  k_trsm_panel<<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+        MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
+                  k_trsm_panel<<<ntile * strips, 256, 0, s>>>((double *)dT, N, NB, MT, k,
+                                                             (const double *)Winv));
     }
     return 0;
 }
 
-// trailing-update engine: 0 = FP64 VALU, 1 = FP64 MFMA (default)
-int g_update_kernel = -1;
-
 // Trailing update of step k over a run of tiles: column n_begin from row
 // n_begin + m_off down, then whole columns, ntiles tiles in all (ntiles < 0:
 // through the last column).
-int launch_update_run(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, int n_begin,
+template <class T>
+int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin,
                       int m_off, int64_t ntiles)
 {
     if (n_begin >= MT || n_begin + m_off >= MT) return 0;
@@ -855,35 +941,36 @@ int launch_update_run(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, 
         for (int m = (n == n_begin ? n + m_off : n); m < MT && left > 0; m++, left--) {
             const double tm = tile_rows(N, NB, MT, m);
             fl += (m == n) ? (double)NB * tm * (tm + 1.0) : 2.0 * tm * NB * NB;
-            by += 2.0 * tm * NB * 8.0 + NB * NB * 8.0;
+            by += (2.0 * tm * NB + NB * NB) * sizeof(T);
         }
     if (g_update_kernel < 0) {
         const char *e = getenv("DPLASMA_MI_UPDATE");
         g_update_kernel = (e && !strcmp(e, "valu")) ? 0 : 1;
     }
-    if (g_update_kernel == 1)
-        MI_LAUNCH(c, s, "dpotrf_update", fl, by,
-                  k_update_mfma<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
-    else
+    if (std::is_same<T, double>::value && g_update_kernel == 0)
         MI_LAUNCH(c, s, "dpotrf_update_valu", fl, by,
-                  k_update_valu<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
+                  k_update_valu<<<(unsigned)grid, 256, 0, s>>>((double *)dT, N, NB, MT, k, n_begin, m_off,
+                                                               nsub));
+    else
+        MI_LAUNCH(c, s, KN<T>::update, fl, by,
+                  k_update_mfma<T><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
     return 0;
 }
 
-int launch_update(Ctx *c, hipStream_t s, double *dT, int N, int NB, int MT, int k, int n_begin,
-                  int n_end)
+template <class T>
+int launch_update(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin, int n_end)
 {
     if (n_begin >= n_end) return 0;
     int64_t ntiles = 0;
     for (int n = n_begin; n < n_end; n++) ntiles += MT - n;
-    return launch_update_run(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles);
+    return launch_update_run<T>(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles);
 }
 
-
+template <class T>
 int prepare_potrf(int NB)
 {
     int RC;
-    int lds = potrf_lds_bytes(NB, &RC);
+    int lds = potrf_lds_bytes<T>(NB, &RC);
     int dev = 0;
     MI_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -895,13 +982,14 @@ int prepare_potrf(int NB)
         return -4;
     }
     if (lds > 65536)
-        MI_CHECK(hipFuncSetAttribute(g_panel_kernel == 1 ? (const void *)k_potrf_tile : (const void *)k_potrf_diag,
+        MI_CHECK(hipFuncSetAttribute(!valu_panel<T>() ? (const void *)k_potrf_tile<T> : (const void *)k_potrf_diag,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     return 0;
 }
 
+// diagonal-block inverses handed from POTRF(k) to TRSM(k): two steps in flight
 struct Workspace {
-    double *winv = nullptr;
+    void *winv = nullptr;
     size_t bytes = 0;
 };
 Workspace g_ws;
@@ -915,6 +1003,105 @@ int ensure_ws(size_t bytes)
     MI_CHECK(hipMalloc(&g_ws.winv, bytes));
     g_ws.bytes = bytes;
     return 0;
+}
+
+// Whole factorization of the packed lower tiles dT (N, NB), resident.
+//   for k: SYRK(k, k+1) -> POTRF(k+1) and the rest of column k+1 -> TRSM(k+1)
+//   on the panel stream, the rest of the trailing update of step k on the
+//   main stream (src/zpotrf_L.jdf, one-panel look-ahead).
+template <class T>
+int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info)
+{
+    *info = 0;
+    if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
+    if (N == 0) return 0;
+    const int MT = (N + NB - 1) / NB;
+    int rc = prepare_potrf<T>(NB);
+    if (rc) return rc;
+    const int nblk = (NB + 31) / 32;
+    const size_t wsz = (size_t)nblk * 1024;        // elements per step
+    if ((rc = ensure_ws(2 * wsz * sizeof(T)))) return rc;
+    T *ws = (T *)g_ws.winv;
+    c->ev_next = 0;
+
+    MI_CHECK(hipEventRecord(c->t0, c->s_main));
+    k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
+    hipEvent_t start = next_event(c);
+    MI_CHECK(hipEventRecord(start, c->s_main));
+    MI_CHECK(hipStreamWaitEvent(c->s_panel, start, 0));
+
+    // panel 0
+    if ((rc = launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, 0, ws))) return rc;
+    if ((rc = launch_trsm<T>(c, c->s_panel, dT, N, NB, MT, 0, ws))) return rc;
+    hipEvent_t pk = next_event(c);
+    MI_CHECK(hipEventRecord(pk, c->s_panel));
+
+    for (int k = 0; k < MT - 1; k++) {
+        MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
+        // the next diagonal tile first: POTRF(k+1) only waits for its SYRK
+        if ((rc = launch_update_run<T>(c, c->s_main, dT, N, NB, MT, k, k + 1, 0, 1))) return rc;
+        hipEvent_t dk = next_event(c);
+        MI_CHECK(hipEventRecord(dk, c->s_main));
+        // then the rest of column k+1, which gates TRSM(k+1)
+        if ((rc = launch_update_run<T>(c, c->s_main, dT, N, NB, MT, k, k + 1, 1, MT - k - 2))) return rc;
+        hipEvent_t ck = next_event(c);
+        MI_CHECK(hipEventRecord(ck, c->s_main));
+        MI_CHECK(hipStreamWaitEvent(c->s_panel, dk, 0));
+        T *w = ws + ((k + 1) & 1) * wsz;
+        if ((rc = launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
+        MI_CHECK(hipStreamWaitEvent(c->s_panel, ck, 0));
+        if ((rc = launch_trsm<T>(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
+        pk = next_event(c);
+        MI_CHECK(hipEventRecord(pk, c->s_panel));
+        // the rest of the trailing matrix overlaps the panel above
+        if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 2, MT))) return rc;
+    }
+    MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
+    MI_CHECK(hipEventRecord(c->t1, c->s_main));
+    int h_info = 0;
+    MI_CHECK(hipMemcpyAsync(&h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
+    if ((rc = mi_synchronize(static_cast<mi_ctx_t *>(c)))) return rc;
+    float ms = 0.f;
+    MI_CHECK(hipEventElapsedTime(&ms, c->t0, c->t1));
+    c->last_ms = ms;
+    *info = h_info;
+    return 0;
+}
+
+// single task-class launches, blocking (tests)
+template <class T>
+int step_potrf(mi_ctx_t *c, int N, int NB, T *dT, int k, int *info)
+{
+    if (N <= 0) { *info = 0; return 0; }
+    const int MT = (N + NB - 1) / NB;
+    int rc = prepare_potrf<T>(NB);
+    if (rc) return rc;
+    const int nblk = (NB + 31) / 32;
+    if ((rc = ensure_ws((size_t)2 * nblk * 1024 * sizeof(T)))) return rc;
+    k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
+    if ((rc = launch_potrf<T>(c, c->s_main, dT, N, NB, MT, k, (T *)g_ws.winv))) return rc;
+    MI_CHECK(hipMemcpyAsync(info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
+    return mi_synchronize(c);
+}
+
+template <class T>
+int step_trsm(mi_ctx_t *c, int N, int NB, T *dT, int k)
+{
+    // the diagonal-block inverses come from the POTRF launch of the same k
+    const int MT = (N + NB - 1) / NB;
+    int rc = launch_trsm<T>(c, c->s_main, dT, N, NB, MT, k, (const T *)g_ws.winv);
+    if (rc) return rc;
+    return mi_synchronize(c);
+}
+
+template <class T>
+int step_update(mi_ctx_t *c, int N, int NB, T *dT, int k, int n_begin, int n_end)
+{
+    const int MT = (N + NB - 1) / NB;
+    if (k < 0 || n_begin <= k || n_end > MT) { set_error("bad update range"); return -2; }
+    int rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, n_begin, n_end);
+    if (rc) return rc;
+    return mi_synchronize(c);
 }
 
 }  // namespace
@@ -937,91 +1124,42 @@ int mi_set_update_engine(int which)
 
 int mi_dpotrf_step_potrf(mi_ctx_t *c, int N, int NB, double *dT, int k, int *info)
 {
-    if (N <= 0) { *info = 0; return 0; }
-    const int MT = (N + NB - 1) / NB;
-    int rc = prepare_potrf(NB);
-    if (rc) return rc;
-    const int nblk = (NB + 31) / 32;
-    if ((rc = ensure_ws((size_t)2 * nblk * 1024 * sizeof(double)))) return rc;
-    k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
-    if ((rc = launch_potrf(c, c->s_main, dT, N, NB, MT, k, g_ws.winv))) return rc;
-    MI_CHECK(hipMemcpyAsync(info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
-    return mi_synchronize(c);
+    return step_potrf<double>(c, N, NB, dT, k, info);
 }
 
 int mi_dpotrf_step_trsm(mi_ctx_t *c, int N, int NB, double *dT, int k)
 {
-    // the diagonal-block inverses come from the POTRF launch of the same k
-    const int MT = (N + NB - 1) / NB;
-    int rc = launch_trsm(c, c->s_main, dT, N, NB, MT, k, g_ws.winv);
-    if (rc) return rc;
-    return mi_synchronize(c);
+    return step_trsm<double>(c, N, NB, dT, k);
 }
 
 int mi_dpotrf_step_update(mi_ctx_t *c, int N, int NB, double *dT, int k, int n_begin, int n_end)
 {
-    const int MT = (N + NB - 1) / NB;
-    if (k < 0 || n_begin <= k || n_end > MT) { set_error("bad update range"); return -2; }
-    int rc = launch_update(c, c->s_main, dT, N, NB, MT, k, n_begin, n_end);
-    if (rc) return rc;
-    return mi_synchronize(c);
+    return step_update<double>(c, N, NB, dT, k, n_begin, n_end);
+}
+
+int mi_spotrf_step_potrf(mi_ctx_t *c, int N, int NB, float *dT, int k, int *info)
+{
+    return step_potrf<float>(c, N, NB, dT, k, info);
+}
+
+int mi_spotrf_step_trsm(mi_ctx_t *c, int N, int NB, float *dT, int k)
+{
+    return step_trsm<float>(c, N, NB, dT, k);
+}
+
+int mi_spotrf_step_update(mi_ctx_t *c, int N, int NB, float *dT, int k, int n_begin, int n_end)
+{
+    return step_update<float>(c, N, NB, dT, k, n_begin, n_end);
 }
 
 int mi_dpotrf_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
 {
-    *info = 0;
-    if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
-    if (N == 0) return 0;
-    const int MT = (N + NB - 1) / NB;
-    int rc = prepare_potrf(NB);
-    if (rc) return rc;
-    const int nblk = (NB + 31) / 32;
-    const size_t wsz = (size_t)nblk * 1024;        // doubles per step
-    if ((rc = ensure_ws(2 * wsz * sizeof(double)))) return rc;
-    c->ev_next = 0;
+    return potrf_tiles<double>(c, N, NB, dT, info);
+}
 
-    MI_CHECK(hipEventRecord(c->t0, c->s_main));
-    k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
-    hipEvent_t start = next_event(c);
-    MI_CHECK(hipEventRecord(start, c->s_main));
-    MI_CHECK(hipStreamWaitEvent(c->s_panel, start, 0));
-
-    // panel 0
-    if ((rc = launch_potrf(c, c->s_panel, dT, N, NB, MT, 0, g_ws.winv))) return rc;
-    if ((rc = launch_trsm(c, c->s_panel, dT, N, NB, MT, 0, g_ws.winv))) return rc;
-    hipEvent_t pk = next_event(c);
-    MI_CHECK(hipEventRecord(pk, c->s_panel));
-
-    for (int k = 0; k < MT - 1; k++) {
-        MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
-        // the next diagonal tile first: POTRF(k+1) only waits for its SYRK
-        if ((rc = launch_update_run(c, c->s_main, dT, N, NB, MT, k, k + 1, 0, 1))) return rc;
-        hipEvent_t dk = next_event(c);
-        MI_CHECK(hipEventRecord(dk, c->s_main));
-        // then the rest of column k+1, which gates TRSM(k+1)
-        if ((rc = launch_update_run(c, c->s_main, dT, N, NB, MT, k, k + 1, 1, MT - k - 2))) return rc;
-        hipEvent_t ck = next_event(c);
-        MI_CHECK(hipEventRecord(ck, c->s_main));
-        MI_CHECK(hipStreamWaitEvent(c->s_panel, dk, 0));
-        double *w = g_ws.winv + ((k + 1) & 1) * wsz;
-        if ((rc = launch_potrf(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
-        MI_CHECK(hipStreamWaitEvent(c->s_panel, ck, 0));
-        if ((rc = launch_trsm(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
-        pk = next_event(c);
-        MI_CHECK(hipEventRecord(pk, c->s_panel));
-        // the rest of the trailing matrix overlaps the panel above
-        if ((rc = launch_update(c, c->s_main, dT, N, NB, MT, k, k + 2, MT))) return rc;
-    }
-    MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
-    MI_CHECK(hipEventRecord(c->t1, c->s_main));
-    int h_info = 0;
-    MI_CHECK(hipMemcpyAsync(&h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
-    if ((rc = mi_synchronize(c))) return rc;
-    float ms = 0.f;
-    MI_CHECK(hipEventElapsedTime(&ms, c->t0, c->t1));
-    c->last_ms = ms;
-    *info = h_info;
-    return 0;
+int mi_spotrf_tiles(mi_ctx_t *c, int N, int NB, float *dT, int *info)
+{
+    return potrf_tiles<float>(c, N, NB, dT, info);
 }
 
 }  // extern "C"

@@ -4,12 +4,15 @@
 // and are processed in chunks of up to 16 columns.
 //
 // Per tile step k (forward; backward mirrors it):
-//   k_trsv_fwd   X_k = L_kk^{-1} B_k            (ztrsm task, LLN.jdf:29-60)
-//   k_gemv_fwd   B_m -= L_mk X_k, m > k         (zgemm task, LLN.jdf:79-174)
+//   k_trsv<NR, false>  X_k = L_kk^{-1} B_k       (ztrsm task, LLN.jdf:29-60)
+//   k_gemv_fwd<NR>     B_m -= L_mk X_k, m > k    (zgemm task, LLN.jdf:79-174)
+// NR = 1 for a single right-hand side, 16 for a chunk of 2..16.
 // The diagonal solve walks 32-row blocks and applies the inverse of each
 // 32x32 diagonal block (computed for all k by k_inv_diag_blocks); the update
 // streams each L tile from HBM exactly once per sweep, which is the bound of
 // this HBM-bound phase.
+#include <algorithm>
+
 #include "mi_common.hpp"
 #include "dplasma_mi.h"
 
@@ -17,235 +20,327 @@ using namespace mi;
 
 namespace {
 
-constexpr int RHSB = 16;   // right-hand-side chunk
+constexpr int RHSB = 16;         // right-hand-side chunk
+constexpr int TSV_THREADS = 512; // diagonal solve: one block, one row (or column) per thread
+constexpr int TSV_MAX_NB = 960;  // two rows per thread, and the 16-RHS LDS budget
+constexpr int GV_ROWS = 64;      // GEMV strip / column group
 
 // Inverse of every 32x32 diagonal block of every diagonal tile.
 // grid = MT * nblk, 32 threads (thread j solves L w = e_j).
-__global__ __launch_bounds__(32) void k_inv_diag_blocks(const double *__restrict__ T, int N, int NB,
-                                                       int MT, int unit, double *__restrict__ W)
+// W(i, j) of block cb of tile k is at W[(k * nblk + cb) * 1024 + j * 32 + i].
+template <class T>
+__global__ __launch_bounds__(32) void k_inv_diag_blocks(const T *__restrict__ Tt, int N, int NB,
+                                                       int MT, int unit, T *__restrict__ W)
 {
     const int nblk = (NB + 31) / 32;
     const int k = blockIdx.x / nblk, cb = blockIdx.x % nblk;
     const int n = tile_rows(N, NB, MT, k);
     const int c0 = cb * 32;
-    double *Wb = W + ((int64_t)k * nblk + cb) * 1024;
+    T *Wb = W + ((int64_t)k * nblk + cb) * 1024;
     const int j = threadIdx.x;
     if (c0 >= n) {                        // padding block: identity
-        for (int i = 0; i < 32; i++) Wb[j * 32 + i] = (i == j) ? 1.0 : 0.0;
+        for (int i = 0; i < 32; i++) Wb[j * 32 + i] = (i == j) ? T(1) : T(0);
         return;
     }
     const int jw = min(32, n - c0);
-    const double *L = T + tile_index(MT, k, k) * (int64_t)NB * NB + (int64_t)c0 * NB + c0;
-    __shared__ double Ls[32][33];
-    for (int c = 0; c < 32; c++) Ls[c][j] = (j < jw && c < jw) ? L[(int64_t)c * NB + j] : 0.0;
+    const T *L = Tt + tile_index(MT, k, k) * (int64_t)NB * NB + (int64_t)c0 * NB + c0;
+    __shared__ T Ls[32][33];
+    for (int c = 0; c < 32; c++) Ls[c][j] = (j < jw && c < jw) ? L[(int64_t)c * NB + j] : T(0);
     __syncthreads();
-    double w[32];
+    T w[32];
 #pragma unroll
     for (int i = 0; i < 32; i++) {
         if (i < jw) {
-            double s = (i == j) ? 1.0 : 0.0;
+            T s = (i == j) ? T(1) : T(0);
 #pragma unroll
             for (int p = 0; p < 32; p++)
                 if (p < i) s -= Ls[p][i] * w[p];
-            w[i] = (i >= j) ? (unit ? s : s / Ls[i][i]) : 0.0;
+            w[i] = (i >= j) ? (unit ? s : s / Ls[i][i]) : T(0);
         } else {
-            w[i] = (i == j) ? 1.0 : 0.0;
+            w[i] = (i == j) ? T(1) : T(0);
         }
     }
 #pragma unroll
     for (int i = 0; i < 32; i++) Wb[j * 32 + i] = w[i];
 }
 
-// Forward diagonal solve of tile k: X_k = L_kk^{-1} B_k for nr <= 16 columns.
-// 256 threads: r = t % 32 (row in block), s = t / 32 (k-slice).
-__global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ T, int N, int NB, int MT,
-                                                 int k, const double *__restrict__ W,
-                                                 double *__restrict__ B, int ldb, int nr)
+// Diagonal solve of tile k for nr <= NR right-hand sides, one block of 512
+// threads.  The tile is walked in 32-row blocks cb; thread t owns rows
+// t + 512 q (q < RPT) of the right-hand side: in registers for NR = 1, in
+// LDS rows of stride NR + 1 for NR = 16.  Per block:
+//   publish W = inv(L_cb,cb) (and, NR = 1, y of the block's rows) to LDS, sync,
+//   x_cb = W y_cb (forward) or W^T y_cb (backward), written to B, sync,
+//   every owner of a row still to be solved subtracts its 32-column strip of
+//   L times x_cb.
+// The strip of L and the next W block are loaded while the next block is
+// published and solved (RPT = 1), so a block costs LDS traffic and two
+// barriers rather than an HBM round trip.
+// Forward (TRANS = false):  X = L_kk^{-1} B_k, rows r >= c0 + 32 use L(r, c0..c0+31).
+// Backward (TRANS = true):  X = L_kk^{-T} B_k, walked from the last block;
+//   rows c < c0 use L(c0..c0+31, c), which is column c of the tile (contiguous).
+template <class T, int NR, bool TRANS, int RPT>
+__global__ __launch_bounds__(TSV_THREADS) void k_trsv(const T *__restrict__ Tt, int N, int NB, int MT,
+                                                      int k, const T *__restrict__ W,
+                                                      T *__restrict__ B, int ldb, int nr)
 {
-    extern __shared__ double sm[];
-    double *Xs = sm;                        // [NB][RHSB]
-    double *red = sm + NB * RHSB;           // [8][32][RHSB]
-    double *Ys = red + 8 * 32 * RHSB;       // [32][RHSB]
-    const int tid = threadIdx.x, r = tid & 31, s = tid >> 5;
+    constexpr int YS = NR + 1;                            // LDS row stride of y (NR = 16)
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    T *Yl = reinterpret_cast<T *>(sm_raw);                        // [RPT * 512][YS] (NR = 16)
+    __shared__ T Ys[32][NR];
+    __shared__ T Xb[32][NR];
+    __shared__ T Ws[32][33];                         // Ws[i][q] = W(i, q)
+    const int tid = threadIdx.x;
     const int n = tile_rows(N, NB, MT, k);
     const int nblk = (NB + 31) / 32;
-    const double *L = T + tile_index(MT, k, k) * (int64_t)NB * NB;
-    double *Bk = B + (int64_t)k * NB;
-    for (int idx = tid; idx < NB * RHSB; idx += 256) {
-        int i = idx / RHSB, j = idx % RHSB;
-        Xs[idx] = (i < n && j < nr) ? Bk[(int64_t)j * ldb + i] : 0.0;
-    }
-    __syncthreads();
-    for (int cb = 0; cb * 32 < n; cb++) {
-        const int c0 = cb * 32;
-        double part[RHSB];
-#pragma unroll
-        for (int j = 0; j < RHSB; j++) part[j] = 0.0;
-        if (c0 + r < n)
-            for (int p = s; p < c0; p += 8) {
-                const double l = L[(int64_t)p * NB + c0 + r];
-#pragma unroll
-                for (int j = 0; j < RHSB; j++) part[j] += l * Xs[p * RHSB + j];
-            }
-#pragma unroll
-        for (int j = 0; j < RHSB; j++) red[(s * 32 + r) * RHSB + j] = part[j];
-        __syncthreads();
-        for (int idx = tid; idx < 32 * RHSB; idx += 256) {
-            const int rr = idx / RHSB, j = idx % RHSB;
-            double y = Xs[(c0 + rr) * RHSB + j];
-            for (int q = 0; q < 8; q++) y -= red[(q * 32 + rr) * RHSB + j];
-            Ys[rr * RHSB + j] = (c0 + rr < n) ? y : 0.0;
-        }
-        __syncthreads();
-        const double *Wb = W + ((int64_t)k * nblk + cb) * 1024;
-        for (int idx = tid; idx < 32 * RHSB; idx += 256) {
-            const int rr = idx / RHSB, j = idx % RHSB;
-            double x = 0.0;
-            for (int q = 0; q <= rr; q++) x += Wb[q * 32 + rr] * Ys[q * RHSB + j];
-            if (c0 + rr < n) Xs[(c0 + rr) * RHSB + j] = x;
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < NB * RHSB; idx += 256) {
-        int i = idx / RHSB, j = idx % RHSB;
-        if (i < n && j < nr) Bk[(int64_t)j * ldb + i] = Xs[idx];
-    }
-}
-
-// B_m -= L_mk X_k for m = k+1 .. MT-1.  Grid = (MT-k-1) * strips of 256 rows;
-// thread = one row; X_k staged in LDS.
-__global__ __launch_bounds__(256) void k_gemv_fwd(const double *__restrict__ T, int N, int NB, int MT,
-                                                 int k, double *__restrict__ B, int ldb, int nr)
-{
-    extern __shared__ double Xs[];          // [NB][RHSB]
-    const int strips = (NB + 255) / 256;
-    const int m = k + 1 + blockIdx.x / strips;
-    const int r = (blockIdx.x % strips) * 256 + threadIdx.x;
-    const int rows_m = tile_rows(N, NB, MT, m);
-    const double *Bk = B + (int64_t)k * NB;
-    for (int idx = threadIdx.x; idx < NB * RHSB; idx += 256) {
-        int i = idx / RHSB, j = idx % RHSB;
-        Xs[idx] = (j < nr) ? Bk[(int64_t)j * ldb + i] : 0.0;
-    }
-    __syncthreads();
-    if (r >= rows_m) return;
-    const double *L = T + tile_index(MT, m, k) * (int64_t)NB * NB + r;
-    double acc[RHSB];
-#pragma unroll
-    for (int j = 0; j < RHSB; j++) acc[j] = 0.0;
-    if (nr == 1) {
-        for (int p = 0; p < NB; p++) acc[0] += L[(int64_t)p * NB] * Xs[p * RHSB];
-    } else {
-        for (int p = 0; p < NB; p++) {
-            const double l = L[(int64_t)p * NB];
-#pragma unroll
-            for (int j = 0; j < RHSB; j++) acc[j] += l * Xs[p * RHSB + j];
-        }
-    }
-    double *Bm = B + (int64_t)m * NB + r;
-    for (int j = 0; j < nr; j++) Bm[(int64_t)j * ldb] -= acc[j];
-}
-
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Backward diagonal solve of tile k: X_k = L_kk^{-T} B_k.
-// 256 threads = 4 waves; block rows are walked from the bottom.
-__global__ __launch_bounds__(256) void k_trsv_bwd(const double *__restrict__ T, int N, int NB, int MT,
-                                                 int k, const double *__restrict__ W,
-                                                 double *__restrict__ B, int ldb, int nr)
-{
-    extern __shared__ double sm[];
-    double *Xs = sm;                        // [NB][RHSB]
-    double *Ys = sm + NB * RHSB;            // [32][RHSB]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int n = tile_rows(N, NB, MT, k);
-    const int nblk = (NB + 31) / 32;
-    const double *L = T + tile_index(MT, k, k) * (int64_t)NB * NB;
-    double *Bk = B + (int64_t)k * NB;
-    for (int idx = tid; idx < NB * RHSB; idx += 256) {
-        int i = idx / RHSB, j = idx % RHSB;
-        Xs[idx] = (i < n && j < nr) ? Bk[(int64_t)j * ldb + i] : 0.0;
-    }
-    __syncthreads();
     const int ncb = (n + 31) / 32;
-    for (int cb = ncb - 1; cb >= 0; cb--) {
-        const int c0 = cb * 32, c1 = min(n, c0 + 32);
-        // y_c = b_c - sum_{p >= c1} L(p, c) x_p, one wave per column c
-        for (int cc = wave; cc < 32; cc += 4) {
-            const int c = c0 + cc;
-            double part[RHSB];
+    const T *L = Tt + tile_index(MT, k, k) * (int64_t)NB * NB;
+    const T *Wk = W + (int64_t)k * nblk * 1024;
+    T *Bk = B + (int64_t)k * NB;
+
+    T y[RPT];                                        // NR = 1
 #pragma unroll
-            for (int j = 0; j < RHSB; j++) part[j] = 0.0;
-            if (c < n)
-                for (int p = c1 + lane; p < n; p += 64) {
-                    const double l = L[(int64_t)c * NB + p];
+    for (int q = 0; q < RPT; q++) {
+        const int r = tid + q * TSV_THREADS;
+        if (NR == 1) {
+            y[q] = (r < n && nr > 0) ? Bk[r] : T(0);
+        } else {
+            for (int j = 0; j < NR; j++)
+                Yl[r * YS + j] = (r < n && j < nr) ? Bk[(int64_t)j * ldb + r] : T(0);
+        }
+    }
+
+    // strip of L used after block cb is solved, for the rows this thread owns;
+    // branch-free: every load stays inside the tile, masked rows get 0
+    T Lc[RPT][32];
+    auto load_strip = [&](int cb) {
+        const int c0 = cb * 32;
 #pragma unroll
-                    for (int j = 0; j < RHSB; j++) part[j] += l * Xs[p * RHSB + j];
+        for (int q = 0; q < RPT; q++) {
+            const int r = tid + q * TSV_THREADS;
+            const int rc = min(r, NB - 1);
+            if (!TRANS) {
+                const bool on = r >= c0 + 32 && r < n;
+#pragma unroll
+                for (int c = 0; c < 32; c++) {
+                    const T v = L[(int64_t)min(c0 + c, NB - 1) * NB + rc];
+                    Lc[q][c] = on ? v : T(0);
                 }
-            for (int j = 0; j < nr; j++) {
-                const double t = wave_sum(part[j]);
-                if (lane == 0) Ys[cc * RHSB + j] = (c < n) ? Xs[c * RHSB + j] - t : 0.0;
+            } else {
+                const bool on = r < c0;
+                const T *col = L + (int64_t)rc * NB;
+#pragma unroll
+                for (int p = 0; p < 32; p++) {
+                    const T v = col[min(c0 + p, NB - 1)];
+                    Lc[q][p] = (on && c0 + p < n) ? v : T(0);
+                }
             }
         }
+    };
+
+    int cb = TRANS ? ncb - 1 : 0;
+    load_strip(cb);
+    T w0 = Wk[(int64_t)cb * 1024 + tid], w1 = Wk[(int64_t)cb * 1024 + tid + TSV_THREADS];
+
+    for (int it = 0; it < ncb; it++) {
+        const int c0 = cb * 32;
+        const int nx = TRANS ? cb - 1 : cb + 1;
+        // (a) publish this block's W (and right-hand side)
+        if (NR == 1) {
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                const int r = tid + q * TSV_THREADS;
+                if (r >= c0 && r < c0 + 32) Ys[r - c0][0] = y[q];
+            }
+        } else {
+            // rows of this block were last updated by other threads in (d)
+            __syncthreads();
+            for (int idx = tid; idx < 32 * NR; idx += TSV_THREADS)
+                Ys[idx / NR][idx % NR] = Yl[(c0 + idx / NR) * YS + idx % NR];
+        }
+        Ws[tid & 31][tid >> 5] = w0;                      // W(i, q) at q * 32 + i
+        Ws[tid & 31][(tid >> 5) + 16] = w1;
         __syncthreads();
-        // x = W^T y with W = inv(L_cb,cb): x_r = sum_{q >= r} W(q, r) y_q
-        const double *Wb = W + ((int64_t)k * nblk + cb) * 1024;
-        for (int idx = tid; idx < 32 * RHSB; idx += 256) {
-            const int rr = idx / RHSB, j = idx % RHSB;
-            double x = 0.0;
-            for (int q = rr; q < 32; q++) x += Wb[rr * 32 + q] * Ys[q * RHSB + j];
-            if (c0 + rr < n && j < nr) Xs[(c0 + rr) * RHSB + j] = x;
+        // (b) the next block's W
+        if (it + 1 < ncb) {
+            w0 = Wk[(int64_t)nx * 1024 + tid];
+            w1 = Wk[(int64_t)nx * 1024 + tid + TSV_THREADS];
+        }
+        // (c) x_cb = W y_cb (or W^T y_cb); rows past n have y = 0 and an identity W
+        for (int idx = tid; idx < 32 * NR; idx += TSV_THREADS) {
+            const int i = idx / NR, j = idx % NR;
+            T s = T(0);
+#pragma unroll
+            for (int q = 0; q < 32; q++) s += (TRANS ? Ws[q][i] : Ws[i][q]) * Ys[q][j];
+            Xb[i][j] = s;
+            if (c0 + i < n && j < nr) Bk[(int64_t)j * ldb + c0 + i] = s;
         }
         __syncthreads();
-    }
-    for (int idx = tid; idx < NB * RHSB; idx += 256) {
-        int i = idx / RHSB, j = idx % RHSB;
-        if (i < n && j < nr) Bk[(int64_t)j * ldb + i] = Xs[idx];
+        // (d) rows still to be solved subtract L strip * x_cb (zero strip elsewhere)
+        if (NR == 1) {
+#pragma unroll
+            for (int c = 0; c < 32; c++) {
+                const T xv = Xb[c][0];
+#pragma unroll
+                for (int q = 0; q < RPT; q++) y[q] -= Lc[q][c] * xv;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                const int r = tid + q * TSV_THREADS;
+#pragma unroll 1
+                for (int j = 0; j < NR; j++) {
+                    T s = T(0);
+#pragma unroll
+                    for (int c = 0; c < 32; c++) s += Lc[q][c] * Xb[c][j];
+                    Yl[r * YS + j] -= s;
+                }
+            }
+        }
+        // the next strip loads while the next block is published and solved
+        if (it + 1 < ncb) load_strip(nx);
+        cb = nx;
     }
 }
 
-// B_m -= L_km^T X_k for m = 0 .. k-1.  Grid = k tiles * (NB/16) column groups;
-// each wave reduces 4 columns of L_km (contiguous in HBM) against X_k.
-__global__ __launch_bounds__(256) void k_gemv_bwd(const double *__restrict__ T, int N, int NB, int MT,
-                                                 int k, double *__restrict__ B, int ldb, int nr)
+// Forward update B_m -= L_mk X_k for m = k+1 .. MT-1.
+// Grid = (MT-k-1) * ceil(NB/64) strips of 64 rows; 4 waves split the NB
+// columns of L_mk, lane = row, 16 independent loads in flight per lane;
+// the four partial sums meet in LDS.  X_k is staged in LDS.
+template <class T, int NR>
+__global__ __launch_bounds__(256) void k_gemv_fwd(const T *__restrict__ Tt, int N, int NB, int MT,
+                                                 int k, T *__restrict__ B, int ldb, int nr)
 {
-    extern __shared__ double Xs[];          // [NB][RHSB]
-    const int groups = (NB + 15) / 16;
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    T *sm = reinterpret_cast<T *>(sm_raw);
+    T *Xs = sm;                         // [NB][NR]
+    T *red = sm + (size_t)NB * NR;      // [3][64][NR]
+    const int strips = (NB + GV_ROWS - 1) / GV_ROWS;
+    const int m = k + 1 + blockIdx.x / strips;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = (blockIdx.x % strips) * GV_ROWS + lane;
+    const int rows_m = tile_rows(N, NB, MT, m);
+    const T *Bk = B + (int64_t)k * NB;   // tile k is full (k < m)
+    for (int idx = threadIdx.x; idx < NB * NR; idx += 256) {
+        const int p = idx / NR, j = idx % NR;
+        Xs[idx] = (j < nr) ? Bk[(int64_t)j * ldb + p] : T(0);
+    }
+    __syncthreads();
+    const int pw = (NB + 3) / 4;
+    const int p0 = wave * pw, p1 = min(NB, p0 + pw);
+    T acc[NR];
+#pragma unroll
+    for (int j = 0; j < NR; j++) acc[j] = T(0);
+    if (r < rows_m) {
+        const T *Lr = Tt + tile_index(MT, m, k) * (int64_t)NB * NB + r;
+        int p = p0;
+        for (; p + 16 <= p1; p += 16) {
+            T l[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) l[u] = Lr[(int64_t)(p + u) * NB];
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+#pragma unroll
+                for (int j = 0; j < NR; j++) acc[j] += l[u] * Xs[(p + u) * NR + j];
+        }
+        for (; p < p1; p++) {
+            const T l = Lr[(int64_t)p * NB];
+#pragma unroll
+            for (int j = 0; j < NR; j++) acc[j] += l * Xs[p * NR + j];
+        }
+    }
+    if (wave > 0)
+#pragma unroll
+        for (int j = 0; j < NR; j++) red[((wave - 1) * 64 + lane) * NR + j] = acc[j];
+    __syncthreads();
+    if (wave == 0 && r < rows_m) {
+        T *Bm = B + (int64_t)m * NB + r;
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            const T s = acc[j] + red[lane * NR + j] + red[(64 + lane) * NR + j] + red[(128 + lane) * NR + j];
+            if (j < nr) Bm[(int64_t)j * ldb] -= s;
+        }
+    }
+}
+
+// Backward update B_m -= L_km^T X_k for m = 0 .. k-1.  Grid = k tiles *
+// ceil(NB/64) groups of 64 columns of L_km (= 64 rows of B_m).  Chunks of 64
+// rows of L_km are read coalesced (lane = row), transposed through LDS and
+// reduced per column; the next chunk is loaded while the current one is used.
+//   NR == 1:  the 4 waves split each chunk's rows, partial sums meet in LDS;
+//   NR == 16: wave w owns right-hand sides 4w .. 4w+3.
+template <class T, int NR>
+__global__ __launch_bounds__(256) void k_gemv_bwd(const T *__restrict__ Tt, int N, int NB, int MT,
+                                                 int k, T *__restrict__ B, int ldb, int nr)
+{
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    T *sm = reinterpret_cast<T *>(sm_raw);
+    const int NBp = (NB + GV_ROWS - 1) / GV_ROWS * GV_ROWS;
+    T *Xs = sm;                          // [NBp][NR]
+    T *Ls = Xs + (size_t)NBp * NR;       // [64][65]: Ls[i * 65 + p]
+    T *red = Ls + 64 * 65;               // [4][64]
+    const int groups = NBp / GV_ROWS;
     const int m = blockIdx.x / groups;
-    const int g = blockIdx.x % groups;
+    const int col0 = (blockIdx.x % groups) * GV_ROWS;
+    const int ncols = min(GV_ROWS, NB - col0);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rows_k = tile_rows(N, NB, MT, k);
-    const double *Bk = B + (int64_t)k * NB;
-    for (int idx = threadIdx.x; idx < NB * RHSB; idx += 256) {
-        int i = idx / RHSB, j = idx % RHSB;
-        Xs[idx] = (i < rows_k && j < nr) ? Bk[(int64_t)j * ldb + i] : 0.0;
+    const T *Bk = B + (int64_t)k * NB;
+    for (int idx = threadIdx.x; idx < NBp * NR; idx += 256) {
+        const int p = idx / NR, j = idx % NR;
+        Xs[idx] = (p < rows_k && j < nr) ? Bk[(int64_t)j * ldb + p] : T(0);
     }
-    __syncthreads();
-    const double *L = T + tile_index(MT, k, m) * (int64_t)NB * NB;
-    for (int cc = 0; cc < 4; cc++) {
-        const int i = g * 16 + wave * 4 + cc;            // column of L_km = row of B_m
-        if (i >= NB) break;
-        double part[RHSB];
+    const T *L = Tt + tile_index(MT, k, m) * (int64_t)NB * NB + (int64_t)col0 * NB;
+    T lr[16];
+    auto load = [&](int p0) {
+        const int p = p0 + lane;
 #pragma unroll
-        for (int j = 0; j < RHSB; j++) part[j] = 0.0;
-        for (int p = lane; p < rows_k; p += 64) {
-            const double l = L[(int64_t)i * NB + p];
-#pragma unroll
-            for (int j = 0; j < RHSB; j++) part[j] += l * Xs[p * RHSB + j];
+        for (int s = 0; s < 16; s++) {
+            const int i = wave + 4 * s;
+            lr[s] = (i < ncols && p < rows_k) ? L[(int64_t)i * NB + p] : T(0);
         }
-        for (int j = 0; j < nr; j++) {
-            const double t = wave_sum(part[j]);
-            if (lane == 0) B[(int64_t)m * NB + i + (int64_t)j * ldb] -= t;
+    };
+    load(0);
+    constexpr int JPW = NR == 1 ? 1 : NR / 4;
+    T acc[JPW];
+#pragma unroll
+    for (int j = 0; j < JPW; j++) acc[j] = T(0);
+    for (int p0 = 0; p0 < rows_k; p0 += 64) {
+#pragma unroll
+        for (int s = 0; s < 16; s++) Ls[(wave + 4 * s) * 65 + lane] = lr[s];
+        __syncthreads();
+        if (p0 + 64 < rows_k) load(p0 + 64);
+        if (NR == 1) {
+#pragma unroll
+            for (int pp = 0; pp < 16; pp++) {
+                const int p = wave * 16 + pp;
+                acc[0] += Ls[lane * 65 + p] * Xs[p0 + p];
+            }
+        } else {
+#pragma unroll 4
+            for (int p = 0; p < 64; p++) {
+                const T l = Ls[lane * 65 + p];
+#pragma unroll
+                for (int jj = 0; jj < JPW; jj++) acc[jj] += l * Xs[(p0 + p) * NR + wave * JPW + jj];
+            }
+        }
+        __syncthreads();
+    }
+    T *Bm = B + (int64_t)m * NB + col0 + lane;  // tile m is full (m < k)
+    if (NR == 1) {
+        red[wave * 64 + lane] = acc[0];
+        __syncthreads();
+        if (wave == 0 && lane < ncols) Bm[0] -= red[lane] + red[64 + lane] + red[128 + lane] + red[192 + lane];
+    } else if (lane < ncols) {
+#pragma unroll
+        for (int jj = 0; jj < JPW; jj++) {
+            const int j = wave * JPW + jj;
+            if (j < nr) Bm[(int64_t)j * ldb] -= acc[jj];
         }
     }
 }
 
-__global__ void k_scale(double *__restrict__ B, int N, int nrhs, int ldb, double alpha)
+template <class T>
+__global__ void k_scale(T *__restrict__ B, int N, int nrhs, int ldb, T alpha)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < (int64_t)N * nrhs) {
@@ -255,7 +350,7 @@ __global__ void k_scale(double *__restrict__ B, int N, int nrhs, int ldb, double
 }
 
 struct SolveWs {
-    double *w = nullptr;
+    void *w = nullptr;
     size_t bytes = 0;
 };
 SolveWs g_sws;
@@ -277,64 +372,144 @@ int set_lds(const void *f, size_t bytes)
     return 0;
 }
 
-}  // namespace
+template <class T> struct SN;
+template <> struct SN<double> {
+    static constexpr const char *diag = "dtrsm_diag", *gemv = "dtrsm_gemv", *inv = "dtrsm_inv_diag",
+                                *scale = "dtrsm_scale";
+};
+template <> struct SN<float> {
+    static constexpr const char *diag = "strsm_diag", *gemv = "strsm_gemv", *inv = "strsm_inv_diag",
+                                *scale = "strsm_scale";
+};
 
-extern "C" int mi_dtrsm_LL(mi_ctx_t *c, int trans, int diag, double alpha, int N, int NB,
-                           const double *dT, int NRHS, double *dB, int ldb)
+template <class T, int NR>
+int solve_chunk(Ctx *c, hipStream_t s, int trans, int N, int NB, int MT, const T *dT,
+                const T *W, T *Bj, int ldb, int nr)
 {
-    if (N <= 0 || NRHS <= 0) return 0;
-    if (NB <= 0 || ldb < N) { set_error("bad sizes"); return -2; }
-    const int MT = (N + NB - 1) / NB;
-    const int nblk = (NB + 31) / 32;
-    int rc;
-    if ((rc = ensure_sws((size_t)MT * nblk * 1024 * sizeof(double)))) return rc;
-    hipStream_t s = c->s_main;
-    MI_CHECK(hipEventRecord(c->t0, s));
-    if (alpha != 1.0) {
-        const int64_t tot = (int64_t)N * NRHS;
-        MI_LAUNCH(c, s, "dtrsm_scale", (double)tot, 16.0 * tot,
-                  k_scale<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(dB, N, NRHS, ldb, alpha));
+    const int strips = (NB + GV_ROWS - 1) / GV_ROWS;
+    const size_t lds_f = ((size_t)NB * NR + 3 * 64 * NR) * sizeof(T);
+    const size_t lds_b = ((size_t)strips * GV_ROWS * NR + 64 * 65 + 256) * sizeof(T);
+    int rc, lim = 0;
+    MI_CHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, 0));
+    const size_t lds_t = (size_t)(NB <= TSV_THREADS ? 1 : 2) * TSV_THREADS * (NR + 1) * sizeof(T) +
+                         (64 * NR + 32 * 33) * sizeof(T);   // diagonal solve, static part included
+    const size_t need = std::max(std::max(lds_f, lds_b), NR == 1 ? (size_t)0 : lds_t);
+    if (need > (size_t)lim) {
+        set_error("tile size NB=%d needs %zu B of LDS for the solve (limit %d)", NB, need, lim);
+        return -4;
     }
-    MI_LAUNCH(c, s, "dtrsm_inv_diag", (double)MT * NB * 32 * 32 / 3.0, (double)MT * NB * 32 * 16.0,
-              k_inv_diag_blocks<<<MT * nblk, 32, 0, s>>>(dT, N, NB, MT, diag == 132, g_sws.w));
-    const size_t lds_x = (size_t)NB * RHSB * sizeof(double);
-    const size_t lds_f = lds_x + (8 * 32 * RHSB + 32 * RHSB) * sizeof(double);
-    const size_t lds_b = lds_x + 32 * RHSB * sizeof(double);
-    if ((rc = set_lds((const void *)k_trsv_fwd, lds_f))) return rc;
-    if ((rc = set_lds((const void *)k_trsv_bwd, lds_b))) return rc;
-    if ((rc = set_lds((const void *)k_gemv_fwd, lds_x))) return rc;
-    if ((rc = set_lds((const void *)k_gemv_bwd, lds_x))) return rc;
-
-    for (int j0 = 0; j0 < NRHS; j0 += RHSB) {
-        const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
-        double *Bj = dB + (int64_t)j0 * ldb;
-        if (trans == 111) {
-            for (int k = 0; k < MT; k++) {
-                const int n = tile_rows(N, NB, MT, k);
-                MI_LAUNCH(c, s, "dtrsm_diag", (double)n * n * nr, (double)n * n * 4.0,
-                          k_trsv_fwd<<<1, 256, lds_f, s>>>(dT, N, NB, MT, k, g_sws.w, Bj, ldb, nr));
-                if (k + 1 < MT) {
-                    const int strips = (NB + 255) / 256;
-                    const double rows = (double)(N - (k + 1) * NB);
-                    MI_LAUNCH(c, s, "dtrsm_gemv", 2.0 * rows * NB * nr, rows * NB * 8.0,
-                              k_gemv_fwd<<<(MT - k - 1) * strips, 256, lds_x, s>>>(dT, N, NB, MT, k,
-                                                                                  Bj, ldb, nr));
-                }
+    // y rows of the diagonal solve (16 right-hand sides): RPT * 512 rows of NR + 1
+    // (the one-RHS kernel keeps its strip and y in registers and exists for
+    // NB <= 512 only; larger tiles take the 16-RHS kernel)
+    constexpr int RPT2 = NR == 1 ? 1 : 2;
+    const int rpt = NB <= TSV_THREADS ? 1 : 2;
+    if (NR == 1 && rpt > 1) { set_error("one-RHS solve needs NB <= %d", TSV_THREADS); return -4; }
+    const size_t lds_y = NR == 1 ? 0 : (size_t)rpt * TSV_THREADS * (NR + 1) * sizeof(T);
+    if (lds_y) {
+        if ((rc = set_lds(rpt == 1 ? (const void *)k_trsv<T, NR, false, 1> : (const void *)k_trsv<T, NR, false, RPT2>, lds_y))) return rc;
+        if ((rc = set_lds(rpt == 1 ? (const void *)k_trsv<T, NR, true, 1> : (const void *)k_trsv<T, NR, true, RPT2>, lds_y))) return rc;
+    }
+    if ((rc = set_lds((const void *)k_gemv_fwd<T, NR>, lds_f))) return rc;
+    if ((rc = set_lds((const void *)k_gemv_bwd<T, NR>, lds_b))) return rc;
+    if (trans == 111) {
+        for (int k = 0; k < MT; k++) {
+            const int n = tile_rows(N, NB, MT, k);
+            MI_LAUNCH(c, s, SN<T>::diag, (double)n * n * nr, (double)n * n * sizeof(T) / 2.0,
+                      (NB <= TSV_THREADS
+                       ? k_trsv<T, NR, false, 1><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)
+                       : k_trsv<T, NR, false, RPT2><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)));
+            if (k + 1 < MT) {
+                const double rows = (double)(N - (k + 1) * NB);
+                MI_LAUNCH(c, s, SN<T>::gemv, 2.0 * rows * NB * nr, rows * NB * sizeof(T),
+                          k_gemv_fwd<T, NR><<<(MT - k - 1) * strips, 256, lds_f, s>>>(dT, N, NB, MT, k, Bj,
+                                                                                     ldb, nr));
             }
-        } else {
-            for (int k = MT - 1; k >= 0; k--) {
-                const int n = tile_rows(N, NB, MT, k);
-                MI_LAUNCH(c, s, "dtrsm_diag", (double)n * n * nr, (double)n * n * 4.0,
-                          k_trsv_bwd<<<1, 256, lds_b, s>>>(dT, N, NB, MT, k, g_sws.w, Bj, ldb, nr));
-                if (k > 0) {
-                    const int groups = (NB + 15) / 16;
-                    const double cols = (double)k * NB;
-                    MI_LAUNCH(c, s, "dtrsm_gemv", 2.0 * cols * n * nr, cols * n * 8.0,
-                              k_gemv_bwd<<<k * groups, 256, lds_x, s>>>(dT, N, NB, MT, k, Bj, ldb, nr));
-                }
+        }
+    } else {
+        for (int k = MT - 1; k >= 0; k--) {
+            const int n = tile_rows(N, NB, MT, k);
+            MI_LAUNCH(c, s, SN<T>::diag, (double)n * n * nr, (double)n * n * sizeof(T) / 2.0,
+                      (NB <= TSV_THREADS
+                       ? k_trsv<T, NR, true, 1><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)
+                       : k_trsv<T, NR, true, RPT2><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)));
+            if (k > 0) {
+                const double cols = (double)k * NB;
+                MI_LAUNCH(c, s, SN<T>::gemv, 2.0 * cols * n * nr, cols * n * sizeof(T),
+                          k_gemv_bwd<T, NR><<<k * strips, 256, lds_b, s>>>(dT, N, NB, MT, k, Bj, ldb, nr));
             }
         }
     }
+    return 0;
+}
+
+}  // namespace
+
+namespace mi {
+
+size_t trsm_ws_elems(int N, int NB)
+{
+    const int MT = (N + NB - 1) / NB;
+    return (size_t)MT * ((NB + 31) / 32) * 1024;
+}
+
+// Inverses of the 32x32 diagonal blocks of every diagonal tile into W
+// (trsm_ws_elems elements), enqueued on s.
+template <class T>
+int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int NB, const T *dT, T *W)
+{
+    const int MT = (N + NB - 1) / NB;
+    const int nblk = (NB + 31) / 32;
+    MI_LAUNCH(c, s, SN<T>::inv, (double)MT * NB * 32 * 32 / 3.0, (double)MT * NB * 32 * 2.0 * sizeof(T),
+              k_inv_diag_blocks<T><<<MT * nblk, 32, 0, s>>>(dT, N, NB, MT, unit, W));
+    return 0;
+}
+
+// B <- op(L)^{-1} B for all NRHS columns, enqueued on s (W from trsm_prepare).
+template <class T>
+int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, const T *W, int NRHS, T *dB,
+                 int ldb)
+{
+    if (NB > TSV_MAX_NB) { set_error("tile size NB=%d above the solve limit %d", NB, TSV_MAX_NB); return -4; }
+    const int MT = (N + NB - 1) / NB;
+    for (int j0 = 0; j0 < NRHS; j0 += RHSB) {
+        const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
+        T *Bj = dB + (int64_t)j0 * ldb;
+        int rc = (nr == 1 && NB <= TSV_THREADS) ? solve_chunk<T, 1>(c, s, trans, N, NB, MT, dT, W, Bj, ldb, nr)
+                                                : solve_chunk<T, RHSB>(c, s, trans, N, NB, MT, dT, W, Bj, ldb, nr);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+template int trsm_prepare<double>(Ctx *, hipStream_t, int, int, int, const double *, double *);
+template int trsm_prepare<float>(Ctx *, hipStream_t, int, int, int, const float *, float *);
+template int trsm_enqueue<double>(Ctx *, hipStream_t, int, int, int, const double *, const double *, int,
+                                  double *, int);
+template int trsm_enqueue<float>(Ctx *, hipStream_t, int, int, int, const float *, const float *, int, float *,
+                                 int);
+
+}  // namespace mi
+
+namespace {
+
+template <class T>
+int trsm_LL(mi_ctx_t *c, int trans, int diag, T alpha, int N, int NB, const T *dT, int NRHS, T *dB, int ldb)
+{
+    if (N <= 0 || NRHS <= 0) return 0;
+    if (NB <= 0 || ldb < N) { set_error("bad sizes"); return -2; }
+    if (NB > TSV_MAX_NB) { set_error("tile size NB=%d above the solve limit %d", NB, TSV_MAX_NB); return -4; }
+    int rc;
+    if ((rc = ensure_sws(trsm_ws_elems(N, NB) * sizeof(T)))) return rc;
+    T *W = (T *)g_sws.w;
+    hipStream_t s = c->s_main;
+    MI_CHECK(hipEventRecord(c->t0, s));
+    if (alpha != T(1)) {
+        const int64_t tot = (int64_t)N * NRHS;
+        MI_LAUNCH(c, s, SN<T>::scale, (double)tot, 2.0 * sizeof(T) * tot,
+                  k_scale<T><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(dB, N, NRHS, ldb, alpha));
+    }
+    if ((rc = trsm_prepare<T>(c, s, diag == 132, N, NB, dT, W))) return rc;
+    if ((rc = trsm_enqueue<T>(c, s, trans, N, NB, dT, W, NRHS, dB, ldb))) return rc;
     MI_CHECK(hipEventRecord(c->t1, s));
     if ((rc = mi_synchronize(c))) return rc;
     float ms = 0.f;
@@ -343,13 +518,37 @@ extern "C" int mi_dtrsm_LL(mi_ctx_t *c, int trans, int diag, double alpha, int N
     return 0;
 }
 
-extern "C" int mi_dpotrs_tiles(mi_ctx_t *c, int N, int NB, const double *dT, int NRHS, double *dB, int ldb)
+template <class T>
+int potrs_tiles(mi_ctx_t *c, int N, int NB, const T *dT, int NRHS, T *dB, int ldb)
 {
-    double ms = 0.0;
-    int rc = mi_dtrsm_LL(c, 111, 131, 1.0, N, NB, dT, NRHS, dB, ldb);
+    int rc = trsm_LL<T>(c, 111, 131, T(1), N, NB, dT, NRHS, dB, ldb);
     if (rc) return rc;
-    ms = c->last_ms;
-    rc = mi_dtrsm_LL(c, 112, 131, 1.0, N, NB, dT, NRHS, dB, ldb);
+    const double ms = c->last_ms;
+    rc = trsm_LL<T>(c, 112, 131, T(1), N, NB, dT, NRHS, dB, ldb);
     c->last_ms += ms;
     return rc;
+}
+
+}  // namespace
+
+extern "C" int mi_dtrsm_LL(mi_ctx_t *c, int trans, int diag, double alpha, int N, int NB,
+                           const double *dT, int NRHS, double *dB, int ldb)
+{
+    return trsm_LL<double>(c, trans, diag, alpha, N, NB, dT, NRHS, dB, ldb);
+}
+
+extern "C" int mi_strsm_LL(mi_ctx_t *c, int trans, int diag, float alpha, int N, int NB,
+                           const float *dT, int NRHS, float *dB, int ldb)
+{
+    return trsm_LL<float>(c, trans, diag, alpha, N, NB, dT, NRHS, dB, ldb);
+}
+
+extern "C" int mi_dpotrs_tiles(mi_ctx_t *c, int N, int NB, const double *dT, int NRHS, double *dB, int ldb)
+{
+    return potrs_tiles<double>(c, N, NB, dT, NRHS, dB, ldb);
+}
+
+extern "C" int mi_spotrs_tiles(mi_ctx_t *c, int N, int NB, const float *dT, int NRHS, float *dB, int ldb)
+{
+    return potrs_tiles<float>(c, N, NB, dT, NRHS, dB, ldb);
 }

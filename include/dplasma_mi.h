@@ -79,6 +79,21 @@ int mi_dplrnt(mi_ctx_t *ctx, int M, int N, double *dA, int lda, int gM, int m0, 
 /* A(i,i) += alpha for i < n (the diagdom option of dplasma_dplrnt,
  * src/zplrnt_wrapper.c:80-93). */
 int mi_dadd_diag(mi_ctx_t *ctx, int n, double *dA, int lda, double alpha);
+/* SPD matrices of prescribed condition for the mixed-precision solve (no
+ * counterpart in the reference, whose dplasma_dlatms, src/zlatms_wrapper.c,
+ * uses a linear spectrum):
+ *   mi_dgauss               n standard Gaussian values: splitmix64 words 2t,
+ *                           2t+1 from seed, Box-Muller;
+ *   mi_dorth_haar           Q (N x N, ld N) = orthogonal factor of the
+ *                           Householder QR of the N x N Gaussian matrix of
+ *                           seed (rocSOLVER dgeqrf + dorgqr, loaded at run
+ *                           time);
+ *   mi_dspd_from_orth_tiles A = Q diag(lambda) Q^T, lambda_j =
+ *                           kappa^(-j/(N-1)), into packed lower tiles
+ *                           (rocBLAS dsyrk; needs 2 N^2 doubles of scratch). */
+int mi_dgauss(mi_ctx_t *ctx, int64_t n, double *dG, uint64_t seed);
+int mi_dorth_haar(mi_ctx_t *ctx, int N, uint64_t seed, double *dQ);
+int mi_dspd_from_orth_tiles(mi_ctx_t *ctx, int N, int NB, const double *dQ, double kappa, double *dT);
 
 /* ------------------------------------------------------------ hot path */
 /* In-place tile Cholesky, lower: the POTRF/TRSM/SYRK/GEMM task classes of
@@ -86,6 +101,11 @@ int mi_dadd_diag(mi_ctx_t *ctx, int n, double *dA, int lda, double alpha);
  * look-ahead.  *info receives k*NB + iinfo of the first non-positive pivot
  * (src/zpotrf_L.jdf:200-202), 0 on success.  Blocking. */
 int mi_dpotrf_tiles(mi_ctx_t *ctx, int N, int NB, double *dT, int *info);
+/* Single precision of the same factorization on FP32 tiles (the "s"
+ * precision of src/zpotrf_wrapper.c:345-371): FP32 MFMA
+ * (v_mfma_f32_16x16x4_f32) trailing update, FP32 diagonal factor and panel
+ * solve, same info rule.  Blocking. */
+int mi_spotrf_tiles(mi_ctx_t *ctx, int N, int NB, float *dT, int *info);
 /* Left/Lower tile triangular solve, B <- alpha op(L)^{-1} B, op = NoTrans
  * (111) or Trans (112), diag NonUnit (131) / Unit (132):
  * src/ztrsm_LLN.jdf, src/ztrsm_LLT.jdf.  Blocking. */
@@ -93,6 +113,10 @@ int mi_dtrsm_LL(mi_ctx_t *ctx, int trans, int diag, double alpha, int N, int NB,
                 const double *dT, int NRHS, double *dB, int ldb);
 /* B <- (L L^T)^{-1} B: src/zpotrs_wrapper.c:91-94.  Blocking. */
 int mi_dpotrs_tiles(mi_ctx_t *ctx, int N, int NB, const double *dT, int NRHS, double *dB, int ldb);
+/* FP32 tiles and right-hand sides: the "s" precision of the two calls above. */
+int mi_strsm_LL(mi_ctx_t *ctx, int trans, int diag, float alpha, int N, int NB,
+                const float *dT, int NRHS, float *dB, int ldb);
+int mi_spotrs_tiles(mi_ctx_t *ctx, int N, int NB, const float *dT, int NRHS, float *dB, int ldb);
 
 /* Single task-class launches (one step k of the factorization), for tests:
  *   POTRF(k)            src/zpotrf_L.jdf:91-208
@@ -109,6 +133,31 @@ int mi_set_update_engine(int which);
 int mi_set_panel_engine(int which);
 int mi_dpotrf_step_trsm(mi_ctx_t *ctx, int N, int NB, double *dT, int k);
 int mi_dpotrf_step_update(mi_ctx_t *ctx, int N, int NB, double *dT, int k, int n_begin, int n_end);
+/* The same three launches on FP32 tiles (the engines do not apply: FP32 is
+ * always MFMA). */
+int mi_spotrf_step_potrf(mi_ctx_t *ctx, int N, int NB, float *dT, int k, int *info);
+int mi_spotrf_step_trsm(mi_ctx_t *ctx, int N, int NB, float *dT, int k);
+int mi_spotrf_step_update(mi_ctx_t *ctx, int N, int NB, float *dT, int k, int n_begin, int n_end);
+
+/* ------------------------------------------------------------ mixed precision */
+/* Mixed-precision solve of A X = B (no counterpart in the reference: the
+ * precision growth path of its dposv, src/zposv_wrapper.c:63-113).  A: FP64
+ * lower tiles of an SPD matrix, overwritten by its FP64 factor only when
+ * *iter < 0; B, X: N x NRHS column-major.  The factor is computed in FP32
+ * (prec 0) and the solution refined in FP64 as LAPACK dsposv does:
+ * *iter >= 0 refinement steps on success; -2 overflow when rounding A,
+ * -3 FP32 factorization failed, -4 the residual stalled, -31 thirty steps
+ * without convergence -- each negative value followed by the FP64
+ * factorization and solve (then *info as mi_dpotrf_tiles).  Blocking. */
+int mi_dsposv_tiles(mi_ctx_t *ctx, int N, int NB, double *dA, int NRHS, const double *dB, int ldb,
+                    double *dX, int ldx, int prec, int *iter, int *info);
+/* R = B - A X in FP64 for the symmetric matrix A held in lower tiles (the
+ * residual of src/dplasma_zcheck.c:179-224 and of the refinement). */
+int mi_dsymm_resid(mi_ctx_t *ctx, int N, int NB, const double *dA, int NRHS, const double *dX, int ldx,
+                   const double *dB, int ldb, double *dR, int ldr);
+/* Round n FP64 values to FP32; *overflow = 1 when a finite value exceeds
+ * FLT_MAX (LAPACK dlag2s).  Blocking. */
+int mi_dlag2s_tiles(mi_ctx_t *ctx, int64_t n, const double *dS, float *dD, int *overflow);
 
 /* ------------------------------------------------------------ checks */
 /* Norms and residuals behind check_dpotrf / check_daxmb
