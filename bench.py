@@ -9,6 +9,9 @@ bracketed by device synchronization; value = LAWN-41 flops / step time.
 
 Also reported: the FP64 solve of the same system (dpotrs, NRHS = 1, b =
 dplrnt(seed 3873)) with the reference's residual ratios computed on the GPU,
+the mixed-precision dposv at N = 32768, NRHS = 16 on A = Q diag(lambda) Q^T
+for kappa in {1e2, 1e5, 1e8} against the FP64 GPU dpotrf + dpotrs of the same
+system (time, refinement steps, backward error, error vs the FP64 answer),
 the roofline of the dominant kernel (the trailing update, timed with HIP
 events on its own stream in a separate profiled step) and the CPU baseline:
 the oracle's tile-DAG executor (a restatement of src/zpotrf_L.jdf with
@@ -72,6 +75,78 @@ def cpu_baseline(seconds_hint=True):
     }
 
 
+def mixed_leg(dp, N, NB, nrhs=16, kappas=(1e2, 1e5, 1e8)):
+    """Mixed-precision dposv at the configuration's shape: A = Q diag(lambda)
+    Q^T (Q Haar from the Householder QR of a Gaussian matrix, seed 7; lambda
+    geometric from 1 to 1/kappa), B = dplrnt(N x 16, seed 3873).  For each
+    kappa: the FP64 GPU path (dpotrf + dpotrs) and the mixed solve with the
+    FP32 factor and with the BF16-split factor, each timed as one blocking
+    call on tiles already in HBM (A copied fresh before each solve, outside
+    the timed region), then checked on the GPU against the original A."""
+    import ctypes
+    import numpy as np
+    mi, _ = dp.libs()
+    eps = 2.0 ** -53
+    out = {"N": N, "NB": NB, "nrhs": nrhs, "cases": []}
+    with dp.Device(0) as dev:
+        MT = (N + NB - 1) // NB
+        nt = MT * (MT + 1) // 2 * NB * NB
+        t0 = time.perf_counter()
+        dQ = dev.alloc(N * N * 8)
+        dev.check(mi.mi_dorth_haar(dev.h, N, 7, dQ), "orth")
+        out["orth_s"] = round(time.perf_counter() - t0, 3)
+        dA, dW = dev.alloc(nt * 8), dev.alloc(nt * 8)
+        dB, dX, dX64 = dev.alloc(N * nrhs * 8), dev.alloc(N * nrhs * 8), dev.alloc(N * nrhs * 8)
+        dev.check(mi.mi_dplrnt(dev.h, N, nrhs, dB, N, N, 0, 0, 3873), "B")
+
+        def check(dx):
+            a, b, x, r = (ctypes.c_double() for _ in range(4))
+            dev.check(mi.mi_dcheck_axmb(dev.h, N, NB, dA, nrhs, dB, N, dx, N, ctypes.byref(a), ctypes.byref(b),
+                                        ctypes.byref(x), ctypes.byref(r)), "check")
+            berr = r.value / (a.value * x.value + b.value)
+            return berr, berr / (N * eps)
+
+        info, it = ctypes.c_int(), ctypes.c_int()
+        warmed = set()
+        for kappa in kappas:
+            t0 = time.perf_counter()
+            dev.check(mi.mi_dspd_from_orth_tiles(dev.h, N, NB, dQ, kappa, dA), "spd")
+            case = {"kappa": kappa, "gen_s": round(time.perf_counter() - t0, 3)}
+            dev.check(mi.mi_memcpy_d2d(dev.h, dW, dA, nt * 8), "copy")
+            dev.check(mi.mi_memcpy_d2d(dev.h, dX64, dB, N * nrhs * 8), "copy")
+            t0 = time.perf_counter()
+            dev.check(mi.mi_dpotrf_tiles(dev.h, N, NB, dW, ctypes.byref(info)), "dpotrf")
+            dev.check(mi.mi_dpotrs_tiles(dev.h, N, NB, dW, nrhs, dX64, N), "dpotrs")
+            t64 = time.perf_counter() - t0
+            berr, ratio = check(dX64)
+            case["fp64"] = {"s": round(t64, 4), "info": info.value, "backward_error": berr, "check_daxmb": ratio}
+            x64 = dev.download(dX64, (N * nrhs,))
+            for prec, name in ((0, "fp32"), (1, "bf16x3")):
+                if prec not in warmed:           # first call sizes the solver's workspaces
+                    dev.check(mi.mi_memcpy_d2d(dev.h, dW, dA, nt * 8), "copy")
+                    dev.check(mi.mi_dsposv_tiles(dev.h, N, NB, dW, nrhs, dB, N, dX, N, prec, ctypes.byref(it),
+                                                 ctypes.byref(info)), "dsposv")
+                    warmed.add(prec)
+                dev.check(mi.mi_memcpy_d2d(dev.h, dW, dA, nt * 8), "copy")
+                t0 = time.perf_counter()
+                dev.check(mi.mi_dsposv_tiles(dev.h, N, NB, dW, nrhs, dB, N, dX, N, prec, ctypes.byref(it),
+                                             ctypes.byref(info)), "dsposv")
+                tm = time.perf_counter() - t0
+                berr, ratio = check(dX)
+                x = dev.download(dX, (N * nrhs,))
+                case[name] = {"s": round(tm, 4), "iter": it.value, "info": info.value,
+                              "backward_error": berr, "check_daxmb": ratio,
+                              "err_vs_fp64": float(np.abs(x - x64).max() / np.abs(x64).max()),
+                              "speedup_vs_fp64": round(t64 / tm, 3)}
+            out["cases"].append(case)
+            print(f"[mixed] kappa={kappa:g}: fp64 {t64*1e3:.1f} ms, fp32 {case['fp32']['s']*1e3:.1f} ms "
+                  f"(iter {case['fp32']['iter']}), bf16x3 {case['bf16x3']['s']*1e3:.1f} ms "
+                  f"(iter {case['bf16x3']['iter']})", file=sys.stderr, flush=True)
+        for p in (dQ, dA, dW, dB, dX, dX64):
+            dev.free(p)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +155,9 @@ def main():
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nb", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision dposv leg")
+    ap.add_argument("--mixed-only", action="store_true", help="run only the mixed-precision dposv leg")
+    ap.add_argument("--kappas", default="1e2,1e5,1e8", help="condition numbers of the mixed leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="one untimed factorization, for rocprofv3 runs")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -102,6 +180,10 @@ def main():
         print(json.dumps({"copy_calibration_bytes_per_launch": {"read": 1 << 30, "write": 1 << 30}}))
         return
     N, NB = args.n, args.nb
+    if args.mixed_only:
+        kap = tuple(float(k) for k in args.kappas.split(","))
+        print(json.dumps({"mixed_dposv": mixed_leg(dp, N, NB, kappas=kap)}))
+        return
     if args.profile_only:
         ctx = dp.Context(0)
         A = dp.SymMatrix(ctx, N, NB, dp.LOWER)
@@ -182,7 +264,7 @@ def main():
         "checks": {"check_dpotrf_pass": chk_f == 0, "check_daxmb_pass": chk_s == 0},
         "fp64_dposv_s": round(total / args.steps + t_potrs, 4),
         "dpotrs_ms": round(t_potrs * 1e3, 3),
-        "mixed_dposv_s": None,
+
         "kernels_ms": {k: round(v[1], 3) for k, v in prof.items()},
         "roofline": {
             "bound": "mfma",
@@ -195,11 +277,13 @@ def main():
             "launches": upd[0],
         },
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
     for M in (A, A0, B, X):
         M.free()
     ctx.close()
+    if not args.no_mixed:
+        line["mixed_dposv"] = mixed_leg(dp, N, NB, kappas=tuple(float(k) for k in args.kappas.split(",")))
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))
 
 

@@ -29,6 +29,8 @@ NOTRANS, TRANS, CONJTRANS = 111, 112, 113
 UPPER, LOWER, UPPERLOWER = 121, 122, 123
 NONUNIT, UNIT = 131, 132
 LEFT, RIGHT = 141, 142
+# mixed-precision factor of dplasma_dposv_mixed (dplasma.h extension)
+MIXED_FP32, MIXED_BF16X3 = 0, 1
 
 MATRIX_DOUBLE = 3
 MATRIX_LAPACK, MATRIX_TILE = 0, 1
@@ -129,6 +131,7 @@ def libs():
     _sig(M, "mi_dpotrf_step_update", i, [vp, i, i, vp, i, i, i])
     _sig(M, "mi_set_update_engine", i, [i])
     _sig(M, "mi_set_panel_engine", i, [i])
+    _sig(M, "mi_set_spotrf_update", i, [i])
     _sig(M, "mi_dcheck_potrf", i, [vp, i, i, vp, vp, pd, pd])
     _sig(M, "mi_dcheck_axmb", i, [vp, i, i, vp, i, vp, i, vp, i, pd, pd, pd, pd])
     _sig(M, "mi_prof_enable", i, [vp, i])
@@ -159,6 +162,8 @@ def libs():
     _sig(D, "dplasma_dpotrf_setrecursive", None, [vp, i])
     _sig(D, "dplasma_dpotrs", i, [vp, i, vp, vp])
     _sig(D, "dplasma_dposv", i, [vp, i, vp, vp])
+    _sig(D, "dplasma_dsposv", i, [vp, i, vp, vp, vp, pi])
+    _sig(D, "dplasma_dposv_mixed", i, [vp, i, vp, vp, vp, i, pi])
     _sig(D, "dplasma_dtrsm", i, [vp, i, i, i, i, d, vp, vp])
     _sig(D, "dplasma_dtrsm_New", vp, [i, i, i, i, d, vp, vp])
     _sig(D, "dplasma_dtrsm_Destruct", None, [vp])
@@ -318,6 +323,12 @@ class Context:
 
     def dposv(self, uplo, A, B):
         return self.D.dplasma_dposv(self.ptr, uplo, A.ptr, B.ptr)
+
+    def dposv_mixed(self, uplo, A, B, X, prec=MIXED_FP32):
+        """Mixed-precision solve (dsposv semantics); returns (info, iter)."""
+        it = ctypes.c_int(0)
+        info = self.D.dplasma_dposv_mixed(self.ptr, uplo, A.ptr, B.ptr, X.ptr, prec, ctypes.byref(it))
+        return info, it.value
 
     def dtrsm(self, side, uplo, trans, diag, alpha, A, B):
         return self.D.dplasma_dtrsm(self.ptr, side, uplo, trans, diag, float(alpha), A.ptr, B.ptr)

@@ -67,6 +67,9 @@ template <class T> int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int 
 template <class T>
 int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, const T *W, int NRHS, T *dB,
                  int ldb);
+// R = B - A X for the symmetric A in lower tiles, FP64, enqueued on s (mi_mixed.hip).
+int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS, const double *dX, int ldx,
+                const double *dB, int ldb, double *dR, int ldr);
 
 }  // namespace mi
 

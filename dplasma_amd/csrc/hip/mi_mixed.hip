@@ -1,5 +1,6 @@
 // mi_mixed.hip -- mixed-precision solve of A X = B on gfx950: factor in
-// FP32 (MFMA), refine in FP64.
+// FP32 (FP32 MFMA, or BF16 MFMA with a two-term split for the trailing
+// updates), refine in FP64.
 //
 // The refinement follows LAPACK's dsposv (the same ITER codes): the
 // FP64 matrix is rounded to FP32 tiles (overflow -> ITER = -2), factored by
@@ -232,12 +233,36 @@ __global__ __launch_bounds__(256) void k_sym_rowabs(const double *__restrict__ A
 
 __global__ void k_flag_clear(int *p) { *p = 0; }
 
-template <class T> T *dev_alloc(size_t n)
-{
+// HIP check that records the error and jumps to the cleanup label
+#define MX_CHECK(call, label)                                                     \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            mi::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,              \
+                          hipGetErrorString(e_));                                 \
+            rc = -1000 - (int)e_;                                                 \
+            goto label;                                                           \
+        }                                                                         \
+    } while (0)
+
+// Workspaces of the mixed solve, kept across calls (grown, never shrunk) so
+// that a repeated solve allocates nothing: FP32 tiles, FP32 right-hand sides,
+// FP64 residual + norms + row sums, FP32 diagonal-block inverses.
+struct Buf {
     void *p = nullptr;
-    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
-    return (T *)p;
-}
+    size_t bytes = 0;
+    void *get(size_t need)
+    {
+        if (bytes >= need) return p;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipMalloc(&p, need) != hipSuccess) { p = nullptr; return nullptr; }
+        bytes = need;
+        return p;
+    }
+};
+Buf g_l32, g_r32, g_r64, g_w32;
 
 }  // namespace
 
@@ -287,7 +312,8 @@ int mi_dlag2s_tiles(mi_ctx_t *c, int64_t n, const double *dS, float *dD, int *ov
 
 // Mixed-precision solve (dsposv semantics).  dA: FP64 lower tiles of the SPD
 // matrix (overwritten by its FP64 factor only when *iter < 0); dB, dX: N x
-// NRHS column-major.  prec: 0 = FP32 factor.
+// NRHS column-major.  prec: 0 = FP32 factor, 1 = FP32 factor whose trailing
+// updates run on BF16 MFMA with the two-term split.
 int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const double *dB, int ldb, double *dX,
                     int ldx, int prec, int *iter, int *info)
 {
@@ -295,7 +321,7 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
     *info = 0;
     if (N <= 0 || NRHS <= 0) return 0;
     if (NB <= 0 || ldb < N || ldx < N) { set_error("bad sizes"); return -2; }
-    if (prec != 0) { set_error("mixed precision %d not available", prec); return -2; }
+    if (prec != 0 && prec != 1) { set_error("mixed precision must be 0 (FP32) or 1 (BF16 split)"); return -2; }
     const int MT = (N + NB - 1) / NB;
     const int64_t ntile_el = (int64_t)MT * (MT + 1) / 2 * NB * NB;
     hipStream_t s = c->s_main;
@@ -303,13 +329,11 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
     const double eps = 0x1p-53;
     int rc = 0;
 
-    float *L32 = dev_alloc<float>((size_t)ntile_el);
-    float *R32 = dev_alloc<float>((size_t)N * NRHS);
-    double *R = dev_alloc<double>((size_t)N * NRHS + 2 * NRHS + (size_t)N);
-    if (!L32 || !R32 || !R) {
-        if (L32) (void)hipFree(L32);
-        if (R32) (void)hipFree(R32);
-        if (R) (void)hipFree(R);
+    float *L32 = (float *)g_l32.get((size_t)ntile_el * sizeof(float));
+    float *R32 = (float *)g_r32.get((size_t)N * NRHS * sizeof(float));
+    double *R = (double *)g_r64.get(((size_t)N * NRHS + 2 * NRHS + (size_t)N) * sizeof(double));
+    float *W32 = (float *)g_w32.get(trsm_ws_elems(N, NB) * sizeof(float));
+    if (!L32 || !R32 || !R || !W32) {
         set_error("out of device memory for the mixed-precision solve");
         return -3;
     }
@@ -333,19 +357,19 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
     // ||A||inf
     k_sym_rowabs<<<grid_of(N), 256, 0, s>>>(dA, N, NB, MT, rs);
     k_colamax<<<1, 256, 0, s>>>(rs, N, N, nrm);
-    MI_CHECK(hipMemcpyAsync(&anrm, nrm, sizeof(double), hipMemcpyDeviceToHost, s));
+    MX_CHECK(hipMemcpyAsync(&anrm, nrm, sizeof(double), hipMemcpyDeviceToHost, s), done);
     // A -> FP32 tiles
     if ((rc = mi_dlag2s_tiles(c, ntile_el, dA, L32, &ovf))) goto done;
     if (ovf) { rc = fall_back(-2); goto done; }
     {
         int sinfo = 0;
-        if ((rc = mi_spotrf_tiles(c, N, NB, L32, &sinfo))) goto done;
+        mi_set_spotrf_update(prec);
+        rc = mi_spotrf_tiles(c, N, NB, L32, &sinfo);
+        mi_set_spotrf_update(0);
+        if (rc) goto done;
         if (sinfo) { rc = fall_back(-3); goto done; }
     }
     {
-        const size_t wel = trsm_ws_elems(N, NB);
-        float *W32 = dev_alloc<float>(wel);
-        if (!W32) { set_error("out of device memory for the mixed-precision solve"); rc = -3; goto done; }
         auto solve32 = [&]() -> int {
             int r;
             if ((r = trsm_enqueue<float>(c, s, 111, N, NB, L32, W32, NRHS, R32, N))) return r;
@@ -354,18 +378,18 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
         const double cte = anrm * eps * std::sqrt((double)N);
         int it = 0;
         bool conv = false;
-        if ((rc = trsm_prepare<float>(c, s, 0, N, NB, L32, W32))) goto done32;
+        if ((rc = trsm_prepare<float>(c, s, 0, N, NB, L32, W32))) goto done;
         // X0 = L32^-T L32^-1 B
         k_flag_clear<<<1, 1, 0, s>>>(c->d_info);
         k_d2s_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(dB, ldb, R32, N, N, NRHS, c->d_info);
-        if ((rc = solve32())) goto done32;
+        if ((rc = solve32())) goto done;
         k_s2d_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R32, N, dX, ldx, N, NRHS, 0);
         for (it = 0; it <= ITERMAX; it++) {
-            if ((rc = dsymm_resid(c, s, N, NB, dA, NRHS, dX, ldx, dB, ldb, R, N))) goto done32;
+            if ((rc = dsymm_resid(c, s, N, NB, dA, NRHS, dX, ldx, dB, ldb, R, N))) goto done;
             k_colamax<<<NRHS, 256, 0, s>>>(R, N, N, nrm);
             k_colamax<<<NRHS, 256, 0, s>>>(dX, ldx, N, nrm + NRHS);
-            MI_CHECK(hipMemcpyAsync(h.data(), nrm, 2 * NRHS * sizeof(double), hipMemcpyDeviceToHost, s));
-            MI_CHECK(hipStreamSynchronize(s));
+            MX_CHECK(hipMemcpyAsync(h.data(), nrm, 2 * NRHS * sizeof(double), hipMemcpyDeviceToHost, s), done);
+            MX_CHECK(hipStreamSynchronize(s), done);
             double worst = 0.0;
             conv = true;
             for (int j = 0; j < NRHS; j++) {
@@ -378,24 +402,16 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
             best = std::min(best, worst);
             if (stall >= 2 || !(worst < INFINITY)) break;
             k_d2s_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R, N, R32, N, N, NRHS, c->d_info);
-            if ((rc = solve32())) goto done32;
+            if ((rc = solve32())) goto done;
             k_s2d_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R32, N, dX, ldx, N, NRHS, 1);
         }
-        (void)hipFree(W32);
-        W32 = nullptr;
         if (conv) {
             *iter = it;
         } else {
             rc = fall_back(it >= ITERMAX ? -31 : -4);
         }
-        goto done;
-    done32:
-        if (W32) (void)hipFree(W32);
     }
 done:
-    (void)hipFree(L32);
-    (void)hipFree(R32);
-    (void)hipFree(R);
     if (rc) return rc;
     MI_CHECK(hipStreamSynchronize(s));
     c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

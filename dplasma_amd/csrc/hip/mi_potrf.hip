@@ -661,11 +661,26 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
         for (int i = 0; i < 4; i++) {
             const int e = tid + 256 * i;
             const int kk = e >> 6, r = (e & 63) * 2;
-            *(V2 *)&As[buf][kk][r] = ra[i];
+            *(V2 *)&As[buf][kk][r] = V2{-ra[i].x, -ra[i].y};     // -A: the MFMA accumulates C - A B^T
             *(V2 *)&Bs[buf][kk][r] = rb[i];
         }
     };
 
+    // full off-diagonal blocks start from C (loaded with the first operands)
+    // and end with a plain store; the others accumulate -A B^T from zero
+    const int fr0 = lane & 15, fk0 = lane >> 4;
+    const bool pre = full && !((m == n) && (bm == bn));
+    if (pre) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int col = c_base + wn * 64 + j * 16 + fr0;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    acc[i][j][q] = Cp[(int64_t)col * NB + r_base + wm * 64 + i * 16 + MF<T>::row(q, fk0)];
+            }
+    }
     const int nstage = (NB + BK - 1) / BK;
     gload(0);
     lstore(0);
@@ -703,16 +718,166 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int rr = r_base + wm * 64 + i * 16 + MF<T>::row(q, fk);
-                if (rr < rows_m && (!diag || rr >= col))
-                    Cp[(int64_t)col * NB + rr] -= acc[i][j][q];
+                if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
+                else if (rr < rows_m && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
             }
         }
 }
 
 // ---------------------------------------------------------------------------
-// The same trailing update on the FP64 VALU.  On gfx950 v_fma_f64 reaches
-// the 78.6 TF FP64 peak while v_mfma_f64_16x16x4_f64 tops out near half of
-// it, so this is the default path.  Work decomposition as k_update_mfma (one
+// FP32 trailing update on BF16 MFMA with a two-term split of both operands
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation): with a = ah + al, ah =
+// bf16(a), al = bf16(a - ah), each product is taken as
+//     al bh + ah bl + ah bh            (the al bl term, ~2^-16 relative, dropped)
+// so the update carries about 16 significant bits per operand instead of 8,
+// at three BF16 MFMAs per product.  Used by the mixed-precision solve, whose
+// FP64 refinement recovers the rest.  Work decomposition, XCD grouping,
+// SYRK masking and the C-preload epilogue as k_update_mfma; K in chunks of
+// 32, split once per workgroup while staged into LDS as [row][k] bf16 rows
+// (row pitch 40 bf16 = 80 B: 16-byte aligned fragment reads, spread banks).
+// MFMA operand layout: A(i, k) and B(k, j) for lane l in i (resp. j) = l % 16,
+// k = 8 (l / 16) + e, e < 8; result element q at row 4 (l / 16) + q.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+constexpr int XK = 32, XS = XK + 8;
+
+__global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt, int N, int NB, int MT, int k,
+                                                         int n_begin, int m_off, int nsub)
+{
+    __shared__ __align__(16) __bf16 Ah[BM * XS];
+    __shared__ __align__(16) __bf16 Al[BM * XS];
+    __shared__ __align__(16) __bf16 Bh[BM * XS];
+    __shared__ __align__(16) __bf16 Bl[BM * XS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int G8 = (G / 8) * 8;
+    const int L = (bid < G8) ? (bid % 8) * (G8 / 8) + bid / 8 : bid;
+    int m, n, bm, bn;
+    decode_block(L, nsub, n_begin, m_off, MT, &m, &n, &bm, &bn);
+    if (m == n && bm < bn) return;
+    const int rows_m = tile_rows(N, NB, MT, m);
+    const int cols_n = tile_rows(N, NB, MT, n);
+    const int r_base = bm * BM, c_base = bn * BM;
+    if (r_base >= rows_m || c_base >= cols_n) return;
+    const bool full = (r_base + BM <= rows_m) && (c_base + BM <= cols_n) && (NB % XK == 0);
+    const float *Ap = Tt + tile_index(MT, m, k) * (int64_t)NB * NB;
+    const float *Bp = Tt + tile_index(MT, n, k) * (int64_t)NB * NB;
+    float *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
+    const int fr = lane & 15, fk = lane >> 4;
+
+    fx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
+    const bool pre = full && !((m == n) && (bm == bn));
+    if (pre) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int col = c_base + wn * 64 + j * 16 + fr;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    acc[i][j][q] = Cp[(int64_t)col * NB + r_base + wm * 64 + i * 16 + 4 * fk + q];
+            }
+    }
+
+    // staging: thread (rq, kq) = (tid % 32, tid / 32) holds rows rq + 32 i
+    // (i < 4) at k = k0 + 4 kq + u (u < 4) of both operands
+    const int rq = tid & 31, kq = tid >> 5;
+    float ra[4][4], rb[4][4];                    // [i][u]
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int kg = k0 + 4 * kq + u;
+            const float *ac = Ap + (int64_t)kg * NB + r_base + rq;
+            const float *bc = Bp + (int64_t)kg * NB + c_base + rq;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (full) {
+                    ra[i][u] = ac[32 * i];
+                    rb[i][u] = bc[32 * i];
+                } else {
+                    const bool kin = kg < NB;
+                    ra[i][u] = (kin && r_base + rq + 32 * i < rows_m) ? ac[32 * i] : 0.f;
+                    rb[i][u] = (kin && c_base + rq + 32 * i < cols_n) ? bc[32 * i] : 0.f;
+                }
+            }
+        }
+    };
+    auto split_store = [&](__bf16 *H, __bf16 *Lo, const float (&v)[4], int row, float sign) {
+        bf4 h, l;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const float x = sign * v[u];
+            const __bf16 xh = (__bf16)x;
+            h[u] = xh;
+            l[u] = (__bf16)(x - (float)xh);
+        }
+        *(bf4 *)&H[row * XS + 4 * kq] = h;
+        *(bf4 *)&Lo[row * XS + 4 * kq] = l;
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            split_store(Ah, Al, ra[i], rq + 32 * i, -1.f);   // -A: the MFMA accumulates C - A B^T
+            split_store(Bh, Bl, rb[i], rq + 32 * i, 1.f);
+        }
+    };
+
+    const int nstage = (NB + XK - 1) / XK;
+    gload(0);
+    for (int st = 0; st < nstage; st++) {
+        lstore();
+        __syncthreads();
+        if (st + 1 < nstage) gload((st + 1) * XK);
+        bf8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = wm * 64 + i * 16 + fr;
+            ah[i] = *(const bf8 *)&Ah[row * XS + 8 * fk];
+            al[i] = *(const bf8 *)&Al[row * XS + 8 * fk];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = wn * 64 + j * 16 + fr;
+            bh[j] = *(const bf8 *)&Bh[row * XS + 8 * fk];
+            bl[j] = *(const bf8 *)&Bl[row * XS + 8 * fk];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        __syncthreads();
+    }
+
+    const bool diag = (m == n) && (bm == bn);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int col = c_base + wn * 64 + j * 16 + fr;
+            if (col >= cols_n) continue;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = r_base + wm * 64 + i * 16 + 4 * fk + q;
+                if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
+                else if (rr < rows_m && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// The same trailing update on the FP64 VALU (engine 0; the MFMA kernel
+// above is the default: with two resident waves per SIMD both reach about
+// the same FP64 rate on gfx950, and the MFMA kernel needs fewer registers
+// per flop).  Work decomposition as k_update_mfma (one
 // 128x128 block of one tile per workgroup, XCD-grouped); thread (tx, ty) =
 // (t % 16, t / 16) keeps the 8x8 block of rows {32g + 2tx + i} and columns
 // {32h + 2ty + j} (g, h < 4, i, j < 2) in registers.  Every k step reads
@@ -851,8 +1016,10 @@ __global__ void k_fill_int(int *p, int v, int n)
 // ---------------------------------------------------------------------------
 // panel engine: 0 = VALU diagonal factor / solve (FP64 only), 1 = MFMA panel updates (default)
 int g_panel_kernel = 1;
-// trailing-update engine: 0 = FP64 VALU, 1 = MFMA (default; FP32 always MFMA)
+// trailing-update engine: 0 = FP64 VALU, 1 = MFMA (default)
 int g_update_kernel = -1;
+// FP32 trailing update: 0 = FP32 MFMA (default), 1 = BF16 MFMA, two-term split
+int g_supdate = 0;
 
 // profiler names per precision
 template <class T> struct KN;
@@ -865,6 +1032,7 @@ template <> struct KN<float> {
     static constexpr const char *potrf = "spotrf_potrf_diag";
     static constexpr const char *trsm = "spotrf_trsm_panel";
     static constexpr const char *update = "spotrf_update";
+    static constexpr const char *update_bf16 = "spotrf_update_bf16x3";
 };
 
 template <class T> bool valu_panel() { return std::is_same<T, double>::value && g_panel_kernel == 0; }
@@ -951,6 +1119,10 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
         MI_LAUNCH(c, s, "dpotrf_update_valu", fl, by,
                   k_update_valu<<<(unsigned)grid, 256, 0, s>>>((double *)dT, N, NB, MT, k, n_begin, m_off,
                                                                nsub));
+    else if (std::is_same<T, float>::value && g_supdate == 1)
+        MI_LAUNCH(c, s, KN<float>::update_bf16, fl, by,
+                  k_update_bf16x3<<<(unsigned)grid, 256, 0, s>>>((float *)dT, N, NB, MT, k, n_begin, m_off,
+                                                                 nsub));
     else
         MI_LAUNCH(c, s, KN<T>::update, fl, by,
                   k_update_mfma<T><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
@@ -1112,6 +1284,13 @@ int mi_set_panel_engine(int which)
 {
     if (which != 0 && which != 1) { set_error("panel engine must be 0 (VALU) or 1 (MFMA)"); return -2; }
     g_panel_kernel = which;
+    return 0;
+}
+
+int mi_set_spotrf_update(int which)
+{
+    if (which != 0 && which != 1) { set_error("FP32 update must be 0 (FP32 MFMA) or 1 (BF16 split)"); return -2; }
+    g_supdate = which;
     return 0;
 }
 

@@ -9,8 +9,9 @@
 // reference walks a tile column.
 //
 // Checks: check_dpotrf / check_daxmb of src/dplasma_zcheck.c:63-133,179-224
-// computed on the device (R = L L^T - A0 on the lower tiles, infinity norms
-// of symmetric and general matrices by row-sum atomics).
+// computed on the device (R = L L^T - A0 on the lower tiles, R = B - A0 X by
+// the strip residual kernel of mi_mixed.hip, infinity norms of symmetric and
+// general matrices by row-sum atomics).
 #include "mi_common.hpp"
 #include "dplasma_mi.h"
 
@@ -251,39 +252,6 @@ __global__ __launch_bounds__(256) void k_sym_rowsum(const double *__restrict__ A
     }
 }
 
-// r = b - A0 x (A0 symmetric, packed lower tiles), one thread per (tile row
-// element, tile, rhs) contribution with atomics into r.
-__global__ __launch_bounds__(256) void k_symv_resid(const double *__restrict__ A, int N, int NB, int MT,
-                                                   const double *__restrict__ x, int ldx, int nrhs,
-                                                   double *__restrict__ r, int ldr)
-{
-    const int64_t ntiles = (int64_t)MT * (MT + 1) / 2;
-    const int64_t work = ntiles * NB * nrhs;
-    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < work;
-         w += (int64_t)gridDim.x * blockDim.x) {
-        const int j = (int)(w % nrhs);
-        const int64_t w2 = w / nrhs;
-        int64_t t = w2 / NB;
-        const int c = (int)(w2 % NB);
-        int n = 0;
-        while (t >= MT - n) { t -= MT - n; n++; }
-        const int m = n + (int)t;
-        const int rows = tile_rows(N, NB, MT, m), cols = tile_rows(N, NB, MT, n);
-        if (c >= cols) continue;
-        const double *col = A + tile_index(MT, m, n) * (int64_t)NB * NB + (int64_t)c * NB;
-        const double *xj = x + (int64_t)j * ldx;
-        double *rj = r + (int64_t)j * ldr;
-        const double xc = xj[n * NB + c];
-        double dot = 0.0;
-        for (int rr = (m == n ? c : 0); rr < rows; rr++) {
-            const double a = col[rr];
-            atomicAdd(&rj[m * NB + rr], -a * xc);             // A(i, c) x_c
-            if (!(m == n && rr == c)) dot += a * xj[m * NB + rr];  // A(c, i) x_i (mirror)
-        }
-        atomicAdd(&rj[n * NB + c], -dot);
-    }
-}
-
 __global__ __launch_bounds__(256) void k_ge_rowsum(const double *__restrict__ A, int M, int Nc, int lda,
                                                   double *__restrict__ rs)
 {
@@ -335,17 +303,6 @@ __global__ void k_zero(double *p, int64_t n)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0.0;
-}
-
-__global__ void k_copy_cm(const double *__restrict__ S, int lds, double *__restrict__ D, int ldd,
-                          int M, int Nc)
-{
-    const int64_t tot = (int64_t)M * Nc;
-    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < tot;
-         w += (int64_t)gridDim.x * blockDim.x) {
-        const int j = (int)(w / M), i = (int)(w % M);
-        D[(int64_t)j * ldd + i] = S[(int64_t)j * lds + i];
-    }
 }
 
 inline unsigned grid_for(int64_t work, int per = 256)
@@ -481,11 +438,8 @@ int mi_dcheck_axmb(mi_ctx_t *c, int N, int NB, const double *dA0, int NRHS, cons
     double *r = buf, *rs = buf + npad * NRHS;
     hipStream_t s = c->s_main;
     k_zero<<<grid_for(npad * NRHS + npad), 256, 0, s>>>(buf, npad * NRHS + npad);
-    k_copy_cm<<<grid_for((int64_t)N * NRHS), 256, 0, s>>>(db, ldb, r, (int)npad, N, NRHS);
-    const int64_t work = (int64_t)MT * (MT + 1) / 2 * NB * NRHS;
-    MI_LAUNCH(c, s, "symv_resid", 2.0 * N * N * NRHS, (double)N * N * 4.0,
-              k_symv_resid<<<grid_for(work), 256, 0, s>>>(dA0, N, NB, MT, dx, ldx, NRHS, r, (int)npad));
     int rc;
+    if ((rc = dsymm_resid(c, s, N, NB, dA0, NRHS, dx, ldx, db, ldb, r, (int)npad))) goto out;
     k_sym_rowsum<<<grid_for((int64_t)MT * (MT + 1) / 2 * NB), 256, 0, s>>>(dA0, N, NB, MT, rs);
     if ((rc = dmax_of(c, rs, npad, Anorm))) goto out;
     k_ge_rowsum<<<grid_for(N), 256, 0, s>>>(db, N, NRHS, ldb, rs);

@@ -244,3 +244,58 @@ int dplasma_dposv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_ma
     if (r3) return r3;
     return info;
 }
+
+/* Mixed-precision solve with dsposv semantics (extension of dplasma_dposv,
+ * src/zposv_wrapper.c:63-113): A is written back only when the FP64
+ * fallback factored it (*iter < 0); B is read; X receives the solution. */
+int dplasma_dposv_mixed(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A,
+                        parsec_tiled_matrix_t *B, parsec_tiled_matrix_t *X, int prec, int *iter)
+{
+    int it = 0;
+    if (iter) *iter = 0;
+    if (uplo != dplasmaUpper && uplo != dplasmaLower) {
+        dpl_error("dplasma_dposv_mixed", "illegal value of uplo");
+        return -1;
+    }
+    if (prec != DPLASMA_MIXED_FP32 && prec != DPLASMA_MIXED_BF16X3) {
+        dpl_error("dplasma_dposv_mixed", "illegal value of prec");
+        return -1;
+    }
+    if (B->m != A->m || X->m != A->m || X->n != B->n) {
+        dpl_error("dplasma_dposv_mixed", "B and X must be N x NRHS with N the order of A");
+        return -1;
+    }
+    dpl_sym_dev_t s;
+    dpl_ge_dev_t gb, gx;
+    memset(&parsec->last, 0, sizeof parsec->last);
+    int rc = dpl_sym_acquire(parsec, A, uplo, 1, &s, &parsec->last.stage_in_ms);
+    if (rc) return rc;
+    if ((rc = dpl_ge_acquire(parsec, B, 1, &gb, &parsec->last.stage_in_ms))) {
+        dpl_sym_release(parsec, A, &s, 0, NULL);
+        return rc;
+    }
+    if ((rc = dpl_ge_acquire(parsec, X, 0, &gx, &parsec->last.stage_in_ms))) {
+        dpl_ge_release(parsec, B, &gb, 0, NULL);
+        dpl_sym_release(parsec, A, &s, 0, NULL);
+        return rc;
+    }
+    int info = 0;
+    rc = mi_dsposv_tiles(parsec->mi, s.N, s.NB, s.d, gb.N, gb.d, gb.ld, gx.d, gx.ld, prec, &it, &info);
+    if (rc) rc = dpl_mi_fail("dplasma_dposv_mixed", rc);
+    else mi_last_span_ms(parsec->mi, &parsec->last.total_ms);
+    int r1 = dpl_ge_release(parsec, X, &gx, rc == 0, &parsec->last.stage_out_ms);
+    int r2 = dpl_ge_release(parsec, B, &gb, 0, NULL);
+    int r3 = dpl_sym_release(parsec, A, &s, rc == 0 && it < 0, &parsec->last.stage_out_ms);
+    if (iter) *iter = it;
+    if (rc) return rc;
+    if (r1) return r1;
+    if (r2) return r2;
+    if (r3) return r3;
+    return info;
+}
+
+int dplasma_dsposv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A,
+                   parsec_tiled_matrix_t *B, parsec_tiled_matrix_t *X, int *iter)
+{
+    return dplasma_dposv_mixed(parsec, uplo, A, B, X, DPLASMA_MIXED_FP32, iter);
+}

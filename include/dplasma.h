@@ -56,6 +56,21 @@ int dplasma_dpotrs(parsec_context_t *parsec, dplasma_enum_t uplo,
 int dplasma_dposv(parsec_context_t *parsec, dplasma_enum_t uplo,
                   parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B);
 
+/* ---------------- mixed precision (extension: the precision growth path of
+ * dplasma_dposv, src/zposv_wrapper.c:63-113; no counterpart in the reference).
+ * Solves A X = B for SPD A (FP64 descriptors) with a low-precision factor
+ * refined in FP64, with LAPACK dsposv semantics: B unchanged; A unchanged
+ * when *iter >= 0 (refinement steps taken), overwritten by its FP64 factor
+ * when *iter < 0 (-2 overflow rounding A, -3 low-precision factorization
+ * failed, -4 refinement stalled, -31 no convergence in 30 steps).  Returns
+ * info as dplasma_dposv (or -1 for an illegal uplo / mismatched B, X). */
+#define DPLASMA_MIXED_FP32   0   /* FP32 factor on FP32 MFMA */
+#define DPLASMA_MIXED_BF16X3 1   /* FP32 factor, trailing updates on BF16 MFMA, two-term split */
+int dplasma_dsposv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A,
+                   parsec_tiled_matrix_t *B, parsec_tiled_matrix_t *X, int *iter);
+int dplasma_dposv_mixed(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A,
+                        parsec_tiled_matrix_t *B, parsec_tiled_matrix_t *X, int prec, int *iter);
+
 /* ---------------- triangular solve (dplasma_z.h:51,153,221) */
 int dplasma_dtrsm(parsec_context_t *parsec, dplasma_enum_t side, dplasma_enum_t uplo,
                   dplasma_enum_t trans, dplasma_enum_t diag, double alpha,

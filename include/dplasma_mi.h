@@ -138,13 +138,18 @@ int mi_dpotrf_step_update(mi_ctx_t *ctx, int N, int NB, double *dT, int k, int n
 int mi_spotrf_step_potrf(mi_ctx_t *ctx, int N, int NB, float *dT, int k, int *info);
 int mi_spotrf_step_trsm(mi_ctx_t *ctx, int N, int NB, float *dT, int k);
 int mi_spotrf_step_update(mi_ctx_t *ctx, int N, int NB, float *dT, int k, int n_begin, int n_end);
+/* Engine of the FP32 trailing update: 0 = FP32 MFMA (default), 1 = BF16
+ * MFMA with FP32 accumulation on a two-term bf16 split of both operands
+ * (three products, ~16 significant bits).  Process-wide. */
+int mi_set_spotrf_update(int which);
 
 /* ------------------------------------------------------------ mixed precision */
 /* Mixed-precision solve of A X = B (no counterpart in the reference: the
  * precision growth path of its dposv, src/zposv_wrapper.c:63-113).  A: FP64
  * lower tiles of an SPD matrix, overwritten by its FP64 factor only when
  * *iter < 0; B, X: N x NRHS column-major.  The factor is computed in FP32
- * (prec 0) and the solution refined in FP64 as LAPACK dsposv does:
+ * (prec 0: FP32 MFMA; prec 1: trailing updates on BF16 MFMA with the
+ * two-term split) and the solution refined in FP64 as LAPACK dsposv does:
  * *iter >= 0 refinement steps on success; -2 overflow when rounding A,
  * -3 FP32 factorization failed, -4 the residual stalled, -31 thirty steps
  * without convergence -- each negative value followed by the FP64

@@ -190,3 +190,40 @@ def test_resident_path_and_rec(ctx, dp, po):
     assert t["total_ms"] > 0 and t["stage_in_ms"] == 0
     ctx.release_all()
     A.free()
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_dposv_mixed(ctx, dp, po, uplo, prec):
+    """dplasma_dposv_mixed (dsposv semantics) on the dplgsy matrix: refined
+    in a few steps, A and B untouched, X equal to the FP64 oracle solution
+    to FP64 accuracy, and the reference's check_daxmb < 60."""
+    N, NB, nrhs = 1700, 320, 16
+    u = dp.LOWER if uplo == "L" else dp.UPPER
+    A = dp.SymMatrix(ctx, N, NB, u); ctx.dplgsy(float(N), u, A, 3872)
+    B = dp.GeMatrix(ctx, N, nrhs, NB, NB); ctx.dplrnt(0, B, 2354)
+    X = dp.GeMatrix(ctx, N, nrhs, NB, NB)
+    a0 = A.array.copy()
+    b = B.to_numpy().copy()
+    info, it = ctx.dposv_mixed(u, A, B, X, prec)
+    assert info == 0 and 0 <= it <= 5, (info, it)
+    assert np.array_equal(A.array, a0)
+    assert np.array_equal(B.to_numpy(), b)
+    x = X.to_numpy()
+    A0 = po.dplgsy_full(N, float(N), 3872)
+    xo = np.linalg.solve(A0, b)
+    assert np.abs(x - xo).max() <= 1e-13 * np.abs(xo).max()
+    assert po.check_daxmb(po.LOWER, A0, b, x) < 60
+    A.free(); B.free(); X.free()
+
+
+def test_dposv_mixed_not_spd(ctx, dp, po):
+    """A non-SPD matrix: the FP32 factorization fails (iter = -3), the FP64
+    fallback factorization reports info > 0 as dplasma_dpotrf does."""
+    N, NB = 378, 93
+    A = dp.SymMatrix(ctx, N, NB, dp.LOWER); ctx.dplgsy(0.0, dp.LOWER, A, 3872)
+    B = dp.GeMatrix(ctx, N, 1, NB, NB); ctx.dplrnt(0, B, 2354)
+    X = dp.GeMatrix(ctx, N, 1, NB, NB)
+    info, it = ctx.dposv_mixed(dp.LOWER, A, B, X)
+    assert info > 0 and it == -3, (info, it)
+    A.free(); B.free(); X.free()

@@ -231,8 +231,6 @@ def test_dtrsm_LL_kernel(dev, po, N, NB, nrhs, trans, unit, alpha):
 EPS32 = 2.0 ** -24
 
 
-@pytest.mark.xfail(reason="FP32 multi-tile factorization residual too large; per-step FP32 kernels pass",
-                   strict=False)
 @pytest.mark.parametrize("N,NB", [(378, 93), (1000, 256), (1536, 512), (700, 320), (64, 64)])
 def test_spotrf_tiles(dev, po, N, NB):
     """FP32 factorization (FP32 MFMA update, FP32 diagonal factor and panel
@@ -247,7 +245,8 @@ def test_spotrf_tiles(dev, po, N, NB):
     got = dev.download(d, T32.shape, np.float32)
     dev.free(d)
     assert info.value == 0
-    A = po.symtiles_to_full(T32.astype(np.float64), N, NB)
+    S = po.symtiles_to_full(T32.astype(np.float64), N, NB)   # stored (lower) tiles only
+    A = np.tril(S) + np.tril(S, -1).T
     L = np.tril(po.symtiles_to_full(got.astype(np.float64), N, NB))
     anorm = np.abs(A).sum(axis=1).max()
     res = np.abs(A - L @ L.T).sum(axis=1).max() / (anorm * N * EPS32)
@@ -356,3 +355,45 @@ def test_s_trailing_update(dev, po, N, NB):
             else:
                 assert np.abs(g - exp).max() < tol, (m, n)
             assert np.array_equal(tile(got, m, n)[tm:, :], C[tm:, :])
+
+
+@pytest.mark.parametrize("N,NB", [(378, 93), (1024, 256), (1536, 512)])
+def test_s_trailing_update_bf16x3(dev, po, N, NB):
+    """FP32 trailing update on BF16 MFMA with the two-term split: each
+    product carries ~16 significant bits per operand, so the error against
+    the FP64 update of the same FP32 data is bounded by ~NB 2^-16 |a||b|
+    (and is far below what a single BF16 product would give)."""
+    rng = np.random.default_rng(N + 2)
+    MT = (N + NB - 1) // NB
+    T = rng.standard_normal(n_sym(N, NB)).astype(np.float32)
+    d = dev.upload(T)
+    dev.check(dev.M.mi_set_spotrf_update(1), "bf16x3")
+    try:
+        dev.check(dev.M.mi_spotrf_step_update(dev.h, N, NB, d, 0, 1, MT), "supdate")
+    finally:
+        dev.M.mi_set_spotrf_update(0)
+    got = dev.download(d, T.shape, np.float32).astype(np.float64)
+    dev.free(d)
+    T64 = T.astype(np.float64)
+
+    def tile(X, m, n):
+        i = n * MT - n * (n - 1) // 2 + (m - n)
+        return X[i * NB * NB:(i + 1) * NB * NB].reshape((NB, NB), order="F")
+
+    worst = 0.0
+    for n in range(1, MT):
+        for m in range(n, MT):
+            tm, tn = min(NB, N - m * NB), min(NB, N - n * NB)
+            A, B, C = tile(T64, m, 0), tile(T64, n, 0), tile(T64, m, n)
+            exp = C[:tm, :tn] - A[:tm, :] @ B[:tn, :].T
+            g = tile(got, m, n)[:tm, :tn]
+            if m == n:
+                low = np.tril(np.ones((tm, tm), bool))
+                err = np.abs(g[low] - exp[low]).max()
+                assert np.array_equal(g[~low], C[:tm, :tm][~low]), (m, n)
+            else:
+                err = np.abs(g - exp).max()
+            worst = max(worst, err / (np.abs(A).max() * np.abs(B).max()))
+            assert np.array_equal(tile(got, m, n)[tm:, :], C[tm:, :])
+    assert worst < 4 * NB * 2.0 ** -16, worst
+    assert worst > 0.0

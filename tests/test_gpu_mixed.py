@@ -75,13 +75,14 @@ def test_dlag2s_overflow(dev):
     assert ovf.value == 0
 
 
-@pytest.mark.xfail(reason="depends on the FP32 multi-tile factorization under investigation", strict=False)
-@pytest.mark.parametrize("N,NB,nrhs,kappa,expect", [
-    (1000, 128, 16, 1e2, "refined"), (1000, 128, 16, 1e5, "refined"), (1000, 128, 16, 1e8, "fallback"),
-    (700, 93, 3, 1e3, "refined"), (512, 512, 1, 1e2, "refined"),
+@pytest.mark.parametrize("N,NB,nrhs,kappa,prec,expect", [
+    (1000, 128, 16, 1e2, 0, "refined"), (1000, 128, 16, 1e5, 0, "refined"), (1000, 128, 16, 1e8, 0, "fallback"),
+    (700, 93, 3, 1e3, 0, "refined"), (512, 512, 1, 1e2, 0, "refined"),
+    (1000, 128, 16, 1e2, 1, "refined"), (1000, 128, 16, 1e8, 1, "fallback"),
 ])
-def test_dsposv_fp32(dev, po, N, NB, nrhs, kappa, expect):
-    """FP32 factor + FP64 refinement reaches FP64-level backward error
+def test_dsposv(dev, po, N, NB, nrhs, kappa, prec, expect):
+    """FP32 factor (prec 0: FP32 MFMA, 1: BF16 split updates) + FP64
+    refinement reaches FP64-level backward error
     (LAPACK dsposv test: ||b - A x|| <= ||x|| ||A|| eps sqrt(N) per column),
     or falls back to the FP64 factorization when FP32 cannot converge
     (kappa * eps32 >~ 1), with the FP64-quality answer either way."""
@@ -93,7 +94,7 @@ def test_dsposv_fp32(dev, po, N, NB, nrhs, kappa, expect):
     dB = dev.upload(np.asfortranarray(B).reshape(-1, order="F"))
     dX = dev.alloc(N * nrhs * 8)
     it, info = ctypes.c_int(-99), ctypes.c_int(-99)
-    dev.check(dev.M.mi_dsposv_tiles(dev.h, N, NB, dT, nrhs, dB, N, dX, N, 0, ctypes.byref(it),
+    dev.check(dev.M.mi_dsposv_tiles(dev.h, N, NB, dT, nrhs, dB, N, dX, N, prec, ctypes.byref(it),
                                     ctypes.byref(info)), "dsposv")
     X = dev.download(dX, (N * nrhs,)).reshape((N, nrhs), order="F")
     Tafter = dev.download(dT, T.shape)
