@@ -25,6 +25,34 @@ __host__ __device__ inline int tile_rows(int N, int NB, int MT, int m)
     return (m == MT - 1) ? N - m * NB : NB;
 }
 
+typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
+
+// Element-type traits of the 16x16x4 MFMA shapes used here: FP64
+// (v_mfma_f64_16x16x4_f64) and FP32 (v_mfma_f32_16x16x4_f32).  Both take one
+// A and one B element per lane (A(i, k) and B(k, j) in lane 16k + i, resp.
+// 16k + j) and return four elements per lane; row(q, l/16) is the output row
+// of element q (column l % 16), as probed on gfx950.
+template <class T> struct MF;
+template <> struct MF<double> {
+    typedef dx4 v4;
+    typedef double2 v2;
+    __device__ static inline v4 mfma(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int q, int fk) { return 4 * q + fk; }
+};
+template <> struct MF<float> {
+    typedef fx4 v4;
+    typedef float2 v2;
+    __device__ static inline v4 mfma(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int q, int fk) { return 4 * fk + q; }
+};
+
 struct ProfRec {
     const char *name;
     hipEvent_t ev0, ev1;
@@ -67,9 +95,10 @@ template <class T> int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int 
 template <class T>
 int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, const T *W, int NRHS, T *dB,
                  int ldb);
-// R = B - A X for the symmetric A in lower tiles, FP64, enqueued on s (mi_mixed.hip).
+// R = B - A X for the symmetric A in lower tiles, FP64, enqueued on s
+// (mi_mixed.hip); rowabs (N values, may be null) receives the row sums of |A|.
 int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS, const double *dX, int ldx,
-                const double *dB, int ldb, double *dR, int ldr);
+                const double *dB, int ldb, double *dR, int ldr, double *rowabs);
 
 }  // namespace mi
 

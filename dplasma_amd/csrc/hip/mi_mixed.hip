@@ -94,7 +94,8 @@ constexpr int RS_ROWS = 64;
 __global__ __launch_bounds__(256) void k_dsymm_resid(const double *__restrict__ A, int N, int NB, int MT,
                                                     const double *__restrict__ X, int ldx,
                                                     const double *__restrict__ B, int ldb,
-                                                    double *__restrict__ R, int ldr, int nr)
+                                                    double *__restrict__ R, int ldr, int nr,
+                                                    double *__restrict__ rowabs)
 {
     __shared__ double Ls[RS_ROWS][RS_ROWS + 1];
     __shared__ double Xs[RS_ROWS][16];
@@ -158,22 +159,35 @@ __global__ __launch_bounds__(256) void k_dsymm_resid(const double *__restrict__ 
     };
 
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double ra = 0.0;                            // sum |A(i, :)| (wave 0, when asked)
+    const bool want_abs = rowabs != nullptr && wave == 0;
     load(0);
     for (int c = 0; c < nchunk; c++) {
         stage(c);
         __syncthreads();
         if (c + 1 < nchunk) load(c + 1);
+        if (want_abs) {
 #pragma unroll 8
-        for (int kk = 0; kk < RS_ROWS; kk++) {
-            const double a = Ls[lane][kk];
+            for (int kk = 0; kk < RS_ROWS; kk++) {
+                const double a = Ls[lane][kk];
+                ra += fabs(a);
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) acc[jj] += a * Xs[kk][4 * wave + jj];
+                for (int jj = 0; jj < 4; jj++) acc[jj] += a * Xs[kk][4 * wave + jj];
+            }
+        } else {
+#pragma unroll 8
+            for (int kk = 0; kk < RS_ROWS; kk++) {
+                const double a = Ls[lane][kk];
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) acc[jj] += a * Xs[kk][4 * wave + jj];
+            }
         }
         __syncthreads();
     }
     const int i = i0 + lane;
     if (i < rows_m) {
         const int64_t row = (int64_t)m * NB + i;
+        if (want_abs) rowabs[row] = ra;
 #pragma unroll
         for (int jj = 0; jj < 4; jj++) {
             const int j = 4 * wave + jj;
@@ -204,31 +218,6 @@ __global__ __launch_bounds__(256) void k_colamax(const double *__restrict__ V, i
         __syncthreads();
     }
     if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
-}
-
-// ||A||inf of the symmetric matrix held in lower tiles: row sums by strip
-// (same walk as the residual), one value per row into rs.
-__global__ __launch_bounds__(256) void k_sym_rowabs(const double *__restrict__ A, int N, int NB, int MT,
-                                                   double *__restrict__ rs)
-{
-    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (row >= N) return;
-    const int m = (int)(row / NB), i = (int)(row % NB);
-    double s = 0.0;
-    for (int n = 0; n < MT; n++) {
-        const int cols = tile_rows(N, NB, MT, n);
-        if (n < m) {
-            const double *t = A + tile_index(MT, m, n) * (int64_t)NB * NB + i;
-            for (int k = 0; k < cols; k++) s += fabs(t[(int64_t)k * NB]);
-        } else if (n > m) {
-            const double *t = A + tile_index(MT, n, m) * (int64_t)NB * NB + (int64_t)i * NB;
-            for (int k = 0; k < cols; k++) s += fabs(t[k]);
-        } else {
-            const double *t = A + tile_index(MT, m, m) * (int64_t)NB * NB;
-            for (int k = 0; k < cols; k++) s += fabs(k <= i ? t[(int64_t)k * NB + i] : t[(int64_t)i * NB + k]);
-        }
-    }
-    rs[row] = s;
 }
 
 __global__ void k_flag_clear(int *p) { *p = 0; }
@@ -270,7 +259,7 @@ namespace mi {
 
 // R = B - A X for NRHS columns (chunks of 16), enqueued on s.
 int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS, const double *dX, int ldx,
-                const double *dB, int ldb, double *dR, int ldr)
+                const double *dB, int ldb, double *dR, int ldr, double *rowabs)
 {
     const int MT = (N + NB - 1) / NB;
     const int strips = (NB + RS_ROWS - 1) / RS_ROWS;
@@ -279,7 +268,7 @@ int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS
         MI_LAUNCH(c, s, "dsymm_resid", 2.0 * N * (double)N * nr, 2.0 * 8.0 * N * (double)N / 2.0,
                   k_dsymm_resid<<<MT * strips, 256, 0, s>>>(dA, N, NB, MT, dX + (int64_t)j0 * ldx, ldx,
                                                            dB + (int64_t)j0 * ldb, ldb, dR + (int64_t)j0 * ldr,
-                                                           ldr, nr));
+                                                           ldr, nr, j0 == 0 ? rowabs : nullptr));
     }
     return 0;
 }
@@ -293,7 +282,7 @@ int mi_dsymm_resid(mi_ctx_t *c, int N, int NB, const double *dA, int NRHS, const
 {
     if (N <= 0 || NRHS <= 0) return 0;
     if (NB <= 0 || ldx < N || ldb < N || ldr < N) { set_error("bad sizes"); return -2; }
-    int rc = dsymm_resid(c, c->s_main, N, NB, dA, NRHS, dX, ldx, dB, ldb, dR, ldr);
+    int rc = dsymm_resid(c, c->s_main, N, NB, dA, NRHS, dX, ldx, dB, ldb, dR, ldr, nullptr);
     if (rc) return rc;
     return mi_synchronize(c);
 }
@@ -331,15 +320,15 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
 
     float *L32 = (float *)g_l32.get((size_t)ntile_el * sizeof(float));
     float *R32 = (float *)g_r32.get((size_t)N * NRHS * sizeof(float));
-    double *R = (double *)g_r64.get(((size_t)N * NRHS + 2 * NRHS + (size_t)N) * sizeof(double));
+    double *R = (double *)g_r64.get(((size_t)N * NRHS + 2 * NRHS + 1 + (size_t)N) * sizeof(double));
     float *W32 = (float *)g_w32.get(trsm_ws_elems(N, NB) * sizeof(float));
     if (!L32 || !R32 || !R || !W32) {
         set_error("out of device memory for the mixed-precision solve");
         return -3;
     }
-    double *nrm = R + (size_t)N * NRHS;           // [0, NRHS): ||r_j||, [NRHS, 2 NRHS): ||x_j||
-    double *rs = nrm + 2 * NRHS;                  // row sums of |A|
-    std::vector<double> h(2 * NRHS);
+    double *nrm = R + (size_t)N * NRHS;           // [0, NRHS): ||r_j||, [NRHS, 2 NRHS): ||x_j||, ||A||
+    double *rs = nrm + 2 * NRHS + 1;              // row sums of |A|
+    std::vector<double> h(2 * NRHS + 1);
     double anrm = 0.0, best = INFINITY;
     int stall = 0, ovf = 0;
     auto fall_back = [&](int code) -> int {
@@ -354,10 +343,6 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
 
     // span on the host clock: the nested blocking calls use the context's events
     const auto t_start = std::chrono::steady_clock::now();
-    // ||A||inf
-    k_sym_rowabs<<<grid_of(N), 256, 0, s>>>(dA, N, NB, MT, rs);
-    k_colamax<<<1, 256, 0, s>>>(rs, N, N, nrm);
-    MX_CHECK(hipMemcpyAsync(&anrm, nrm, sizeof(double), hipMemcpyDeviceToHost, s), done);
     // A -> FP32 tiles
     if ((rc = mi_dlag2s_tiles(c, ntile_el, dA, L32, &ovf))) goto done;
     if (ovf) { rc = fall_back(-2); goto done; }
@@ -375,7 +360,7 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
             if ((r = trsm_enqueue<float>(c, s, 111, N, NB, L32, W32, NRHS, R32, N))) return r;
             return trsm_enqueue<float>(c, s, 112, N, NB, L32, W32, NRHS, R32, N);
         };
-        const double cte = anrm * eps * std::sqrt((double)N);
+        double cte = 0.0;                         // ||A||inf eps sqrt(N), from the first residual
         int it = 0;
         bool conv = false;
         if ((rc = trsm_prepare<float>(c, s, 0, N, NB, L32, W32))) goto done;
@@ -385,11 +370,19 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
         if ((rc = solve32())) goto done;
         k_s2d_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R32, N, dX, ldx, N, NRHS, 0);
         for (it = 0; it <= ITERMAX; it++) {
-            if ((rc = dsymm_resid(c, s, N, NB, dA, NRHS, dX, ldx, dB, ldb, R, N))) goto done;
+            // the first residual also yields the row sums of |A|, hence ||A||inf
+            if ((rc = dsymm_resid(c, s, N, NB, dA, NRHS, dX, ldx, dB, ldb, R, N, it == 0 ? rs : nullptr)))
+                goto done;
             k_colamax<<<NRHS, 256, 0, s>>>(R, N, N, nrm);
             k_colamax<<<NRHS, 256, 0, s>>>(dX, ldx, N, nrm + NRHS);
-            MX_CHECK(hipMemcpyAsync(h.data(), nrm, 2 * NRHS * sizeof(double), hipMemcpyDeviceToHost, s), done);
+            if (it == 0) k_colamax<<<1, 256, 0, s>>>(rs, N, N, nrm + 2 * NRHS);
+            MX_CHECK(hipMemcpyAsync(h.data(), nrm, (2 * NRHS + 1) * sizeof(double), hipMemcpyDeviceToHost, s),
+                     done);
             MX_CHECK(hipStreamSynchronize(s), done);
+            if (it == 0) {
+                anrm = h[2 * NRHS];
+                cte = anrm * eps * std::sqrt((double)N);
+            }
             double worst = 0.0;
             conv = true;
             for (int j = 0; j < NRHS; j++) {

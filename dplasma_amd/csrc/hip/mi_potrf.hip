@@ -36,33 +36,6 @@ namespace {
 
 constexpr int PJB = 32;            // panel width inside the diagonal factorization
 constexpr int POTRF_THREADS = 512;
-typedef double dx4 __attribute__((ext_vector_type(4)));
-typedef float fx4 __attribute__((ext_vector_type(4)));
-
-// Element-type traits of the 16x16x4 MFMA shapes used here: FP64
-// (v_mfma_f64_16x16x4_f64) and FP32 (v_mfma_f32_16x16x4_f32).  Both take one
-// A and one B element per lane (A(i, k) and B(k, j) in lane 16k + i, resp.
-// 16k + j) and return four elements per lane; row(q, l/16) is the output row
-// of element q (column l % 16), as probed on gfx950.
-template <class T> struct MF;
-template <> struct MF<double> {
-    typedef dx4 v4;
-    typedef double2 v2;
-    __device__ static inline v4 mfma(double a, double b, v4 c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    __device__ static inline int row(int q, int fk) { return 4 * q + fk; }
-};
-template <> struct MF<float> {
-    typedef fx4 v4;
-    typedef float2 v2;
-    __device__ static inline v4 mfma(float a, float b, v4 c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    __device__ static inline int row(int q, int fk) { return 4 * fk + q; }
-};
 
 // ---------------------------------------------------------------------------
 // One wave: acc[ct] += X[r0+i][0:K] * Y[c0+16ct+j][0:K]^T for the 16x16 output
@@ -114,45 +87,6 @@ __device__ inline void mfma_panel_2(const T *__restrict__ X, int ldx, int r0, in
     acc[1] += e1 + o1;
 }
 
-// One wave, up to four 16-row tiles at once: acc[t][ct] += X[r0+128t+i][0:K] *
-// Y[16ct+j][0:K]^T for t < nt (row tiles r0, r0 + 128, r0 + 256, r0 + 384),
-// ct = 0, 1; X and Y share the leading dimension ld (K * ld < 2^31).  All operands of a 32-wide K pass (4 x 8 A values, 2 x 8 B
-// values per lane) are in flight together, so the global-load latency is
-// paid once per pass for the four tiles.  Layout and masking as mfma_panel_2.
-template <class T>
-__device__ inline void mfma_panel_4x2(const T *__restrict__ X, int ld, int r0, int nt, int xr_lim,
-                                      const T *__restrict__ Y, int yr_lim, int K,
-                                      typename MF<T>::v4 acc[4][2])
-{
-    const int lane = threadIdx.x & 63, fr = lane & 15, fk = lane >> 4;
-    const bool yv0 = fr < yr_lim, yv1 = 16 + fr < yr_lim;
-    // wave-uniform row-tile bases, one 32-bit per-lane offset per K slice
-    bool xv[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) xv[t] = t < nt && r0 + 128 * t + fr < xr_lim;
-    const T *Xt0 = X + r0, *Xt1 = X + r0 + 128, *Xt2 = X + r0 + 256, *Xt3 = X + r0 + 384;
-    for (int k = 0; k < K; k += 32) {
-        T a[4][8], b0[8], b1[8];
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int off = (k + 4 * s + fk) * ld + fr;
-            b0[s] = yv0 ? Y[off] : T(0);
-            b1[s] = yv1 ? Y[off + 16] : T(0);
-            a[0][s] = xv[0] ? Xt0[off] : T(0);
-            a[1][s] = xv[1] ? Xt1[off] : T(0);
-            a[2][s] = xv[2] ? Xt2[off] : T(0);
-            a[3][s] = xv[3] ? Xt3[off] : T(0);
-        }
-#pragma unroll
-        for (int s = 0; s < 8; s++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                acc[t][0] = MF<T>::mfma(a[t][s], b0[s], acc[t][0]);
-                acc[t][1] = MF<T>::mfma(a[t][s], b1[s], acc[t][1]);
-            }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Wave-uniform copy of a double held by one lane (two v_readlane_b32).
 __device__ inline double readlane_d(double v, int lane)
@@ -168,18 +102,66 @@ __device__ inline float readlane_t(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// POTRF(k), left-looking by 32-column panels in one workgroup of 8 waves:
-//   P = A[j0:n, j0:j0+32] - L[j0:n, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, LDS P;
-//       wave w owns the 16-row tiles w, w+8, w+16, w+24 and runs them together)
-//   wave 0, one sweep over the 32 columns c:
-//       lanes 0..31:  row r of L_jj = chol(P[0:32])   (right-looking, in registers)
-//       lanes 32..63: column j of W_jj = L_jj^{-1}    (forward substitution)
-//     both updates are the same FMA  v[c2] -= m * L(c2, c)  with the
-//     multiplier m = L(r, c) (resp. W(c, j)) and L(c2, c) broadcast by
-//     v_readlane, so the serial path has no LDS round trip and no branch;
-//     columns past the tile edge get an identity diagonal
-//   P[32:] <- P[32:] W_jj^T                  (MFMA, K = 32)
-// then the panel is written back.  W_jj goes to Winv for the panel TRSM.
+// C(r0:r0+32, c0:c0+32) -= X(r0:, 0:32) X(c0:, 0:32)^T on the lower triangle
+// of A (rows, columns < n), X held in LDS as P[k * RC + row]; one wave, 2 x 2
+// MFMA 16x16 tiles sharing their operands, K = 32.  The MFMA is fed (X_c, X_r)
+// so that it accumulates C^T: lane l then holds rows r0 + 16i + l % 16 of C,
+// and the loads and stores of the column-major C are 128-byte runs.
+template <class T>
+__device__ inline void syrk_unit32(T *__restrict__ A, int ld, int n, const T *__restrict__ P, int RC, int r0,
+                                   int c0, int fr, int fk)
+{
+    typedef typename MF<T>::v4 V4;
+    V4 acc[2][2];                                    // acc[i][j]: rows r0+16i.., columns c0+16j..
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int r = r0 + 16 * i + fr, c = c0 + 16 * j + MF<T>::row(q, fk);
+                acc[i][j][q] = (r < n && c < n && r >= c) ? A[(int64_t)c * ld + r] : T(0);
+            }
+    const int ra0 = min(r0 + fr, RC - 1), ra1 = min(r0 + 16 + fr, RC - 1);
+    const int cb0 = min(c0 + fr, RC - 1), cb1 = min(c0 + 16 + fr, RC - 1);
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int kk = (4 * s + fk) * RC;
+        const T x0 = -P[kk + ra0], x1 = -P[kk + ra1];
+        const T y0 = P[kk + cb0], y1 = P[kk + cb1];
+        acc[0][0] = MF<T>::mfma(y0, x0, acc[0][0]);
+        acc[0][1] = MF<T>::mfma(y1, x0, acc[0][1]);
+        acc[1][0] = MF<T>::mfma(y0, x1, acc[1][0]);
+        acc[1][1] = MF<T>::mfma(y1, x1, acc[1][1]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int r = r0 + 16 * i + fr, c = c0 + 16 * j + MF<T>::row(q, fk);
+                if (r < n && c < n && r >= c) A[(int64_t)c * ld + r] = acc[i][j][q];
+            }
+}
+
+// POTRF(k): right-looking by 32-column blocks in one workgroup of 8 waves,
+// with look-ahead inside the tile.  LDS holds the current block column X
+// (rows j0..n-1, 32 columns) as P[c * RC + row] (absolute rows) and W_jj.
+// Per block j0:
+//   wave 0     L_jj = chol(D), W_jj = L_jj^{-1} in one register sweep over the
+//              32 columns (lanes 0..31: rows of L, lanes 32..63: columns of
+//              W, both by the FMA v[c2] -= m * L(c2, c) with L(c2, c)
+//              broadcast by v_readlane); columns past the tile edge get an
+//              identity diagonal
+//   waves 1-7  meanwhile: the trailing update of the previous block for
+//              columns >= j0 + 32 (A -= X X^T, 32x32 MFMA units from LDS)
+//   all        X = A(j0+32:n, j0:j0+32) W_jj^T on MFMA -> P and A;
+//              the next block column (j0+32 .. j0+63) updated first,
+//   wave 0     then loads the next diagonal block D while waves 1-7 go on
+//              with the rest of the trailing update.
+// Every element is updated in the same order as the right-looking loop, so
+// results do not depend on timing.  W_jj goes to Winv for the panel TRSM.
 // info = k*NB + j of the first non-positive pivot j (1-based), as
 // CORE_dpotrf reports it (src/zpotrf_L.jdf:200-202).
 template <class T>
@@ -189,48 +171,33 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     typedef typename MF<T>::v4 V4;
-    T *smem = reinterpret_cast<T *>(smem_raw);
-    T *P = smem;                    // 32 columns x RC rows (RC odd: no bank aliasing)
-    T *Ws = P + 32 * RC;            // W(i, j) at Ws[j*32 + i]
+    T *P = reinterpret_cast<T *>(smem_raw);          // 32 columns x RC rows (RC odd, >= n + 1)
+    T *Ws = P + 32 * RC;                             // W(i, j) at Ws[j*32 + i]
     __shared__ int s_fail;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
+    const bool lrow = lane < 32;
     if (tid == 0) s_fail = 0;
+
+    // v[c]: lanes < 32 hold D(lane, c) of the current diagonal block, lanes
+    // >= 32 the identity column that becomes W(:, lane - 32)
+    T v[32];
+    auto load_diag = [&](int j0) {
+        int j = lane - 32;
+        asm volatile("" : "+v"(j));                  // built per block: keeps the identity out of LICM
+#pragma unroll
+        for (int c = 0; c < 32; c++) {
+            const int r = j0 + lane, cc = j0 + c;
+            v[c] = lrow ? ((r < n && cc < n) ? A[(int64_t)cc * ld + r] : T(0)) : (c == j ? T(1) : T(0));
+        }
+    };
+    if (wave == 0) load_diag(0);
     __syncthreads();
 
     for (int j0 = 0; j0 < n; j0 += 32) {
-        const int jw = min(32, n - j0), R = n - j0, R16 = (R + 15) >> 4;
-        T *Aj = A + j0;                          // row j0 of column 0
-        for (int g0 = wave; g0 < R16; g0 += 32) {
-            const int nt = min(4, (R16 - g0 + 7) >> 3);     // row tiles g0 + 8t < R16
-            V4 acc[4][2];
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc[t][0] = acc[t][1] = V4{T(0), T(0), T(0), T(0)};
-            if (j0 > 0) mfma_panel_4x2<T>(Aj, ld, g0 * 16, nt, R, Aj, jw, j0, acc);
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                if (t >= nt) break;
-                const int rt = g0 + 8 * t;
-#pragma unroll
-                for (int ct = 0; ct < 2; ct++)
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int r = rt * 16 + MF<T>::row(q, fk), c = ct * 16 + fr;
-                        P[c * RC + r] = (r < R && c < jw) ? Aj[(int64_t)(j0 + c) * ld + r] - acc[t][ct][q] : T(0);
-                    }
-            }
-        }
-        __syncthreads();
-
+        const int jw = min(32, n - j0);
         if (wave == 0) {
-            // v[c]: lanes < 32 hold P(lane, c), lanes >= 32 hold e_{lane-32}(c)
-            T v[32];
-            const bool lrow = lane < 32;
-            int j = lane - 32;
-            asm volatile("" : "+v"(j));             // built per panel: keeps the identity out of LICM
-#pragma unroll
-            for (int c = 0; c < 32; c++) v[c] = lrow ? P[c * RC + lane] : (c == j ? T(1) : T(0));
-            unsigned bad = 0;                       // non-positive pivots, bit c
+            unsigned bad = 0;                        // non-positive pivots, bit c
 #pragma unroll
             for (int c = 0; c < 32; c++) {
                 T d0 = readlane_t(v[c], c);
@@ -239,7 +206,7 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
                 d0 = ok && c < jw ? d0 : T(1);
                 const T d = sqrt(d0);
                 const T rd = T(1) / d;
-                const T sc = v[c] * rd;             // L(lane, c) below the diagonal / W(c, j)
+                const T sc = v[c] * rd;              // L(lane, c) below the diagonal / W(c, j)
                 const T m = (!lrow || lane > c) ? sc : T(0);
                 v[c] = lrow ? (lane == c ? d : (lane > c ? sc : v[c])) : sc;
 #pragma unroll
@@ -248,50 +215,64 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
             if (lrow) {
 #pragma unroll
                 for (int c = 0; c < 32; c++)
-                    if (c <= lane) P[c * RC + lane] = v[c];
+                    if (c <= lane) P[c * RC + j0 + lane] = v[c];
             } else {
 #pragma unroll
-                for (int c = 0; c < 32; c++) Ws[j * 32 + c] = v[c];
+                for (int c = 0; c < 32; c++) Ws[(lane - 32) * 32 + c] = v[c];
             }
             if (bad && lane == 0) s_fail = j0 + __builtin_ctz(bad) + 1;
+        } else if (j0 > 0) {
+            // trailing update of block j0-32 beyond the current block column
+            const int b0 = j0 + 32, nb = (n - b0 + 31) / 32;
+            for (int u = wave - 1; u < nb * (nb + 1) / 2; u += 7) {
+                int bi = 0, rem = u;                 // unit u -> (row block bi >= col block bj)
+                while (rem > bi) { rem -= bi + 1; bi++; }
+                syrk_unit32<T>(A, ld, n, P, RC, b0 + 32 * bi, b0 + 32 * rem, fr, fk);
+            }
         }
         __syncthreads();
         if (s_fail) break;
-        for (int idx = tid; idx < 1024; idx += 512) Winv[(j0 >> 5) * 1024 + idx] = Ws[idx];
 
-        // rows below the diagonal block: X = P2 W^T on MFMA (K = 32)
-        const int R2 = R - 32;
-        for (int rt = wave; rt * 16 < R2; rt += 8) {
+        // X = A(j0+32:n, j0:j0+32) W^T on MFMA (K = 32), accumulated as
+        // X^T = W P2^T so that lane l holds row rt + l % 16; rows to P and A
+        for (int rt = j0 + 32 + 16 * wave; rt < n; rt += 128) {
             V4 e0 = {T(0), T(0), T(0), T(0)}, e1 = e0;
-            const int r = 32 + rt * 16 + fr;
+            const int r = rt + fr;
+            const int rc = min(r, n - 1);
 #pragma unroll
             for (int s4 = 0; s4 < 8; s4++) {
                 const int kk = 4 * s4 + fk;
-                const T a = P[kk * RC + r];                 // P2(r, kk)
-                const T b0 = Ws[kk * 32 + fr];              // W(fr, kk)
-                const T b1 = Ws[kk * 32 + 16 + fr];         // W(16 + fr, kk)
-                e0 = MF<T>::mfma(a, b0, e0);
-                e1 = MF<T>::mfma(a, b1, e1);
+                const T a = (r < n && kk < jw) ? A[(int64_t)(j0 + kk) * ld + rc] : T(0);
+                e0 = MF<T>::mfma(Ws[kk * 32 + fr], a, e0);          // W(c, kk), c = fr
+                e1 = MF<T>::mfma(Ws[kk * 32 + 16 + fr], a, e1);     // c = 16 + fr
             }
-            // every lane's A reads of rows r are done before the stores below
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (r < n) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int rr = 32 + rt * 16 + MF<T>::row(q, fk);
-                if (rr < R) {
-                    P[fr * RC + rr] = e0[q];
-                    P[(16 + fr) * RC + rr] = e1[q];
+                for (int q = 0; q < 4; q++) {
+                    const int c = MF<T>::row(q, fk);
+                    P[c * RC + r] = e0[q];
+                    P[(16 + c) * RC + r] = e1[q];
+                    if (c < jw) A[(int64_t)(j0 + c) * ld + r] = e0[q];
+                    if (16 + c < jw) A[(int64_t)(j0 + 16 + c) * ld + r] = e1[q];
                 }
             }
         }
-        __syncthreads();
-        for (int idx = tid; idx < jw * R; idx += 512) {
-            const int c = idx / R, r = idx - c * R;
-            if (r >= c) Aj[(int64_t)(j0 + c) * ld + r] = P[c * RC + r];
+        // L_jj to A, W_jj to Winv
+        for (int idx = tid; idx < 1024; idx += 512) {
+            const int c = idx >> 5, r = idx & 31;
+            if (r >= c && r < jw && c < jw) A[(int64_t)(j0 + c) * ld + j0 + r] = P[c * RC + j0 + r];
+            Winv[(j0 >> 5) * 1024 + idx] = Ws[idx];
         }
         __syncthreads();
+
+        // the next block column first (rows j0+32.., columns j0+32..j0+63)
+        const int b1 = j0 + 32;
+        if (b1 < n) {
+            for (int u = wave; u < (n - b1 + 31) / 32; u += 8)
+                syrk_unit32<T>(A, ld, n, P, RC, b1 + 32 * u, b1, fr, fk);
+        }
+        __syncthreads();
+        if (wave == 0 && b1 < n) load_diag(b1);
     }
     if (tid == 0 && s_fail) atomicCAS(d_info, 0, info_base + s_fail);
 }
@@ -675,10 +656,10 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int col = c_base + wn * 64 + j * 16 + fr0;
+                const int row = r_base + wm * 64 + i * 16 + fr0;
 #pragma unroll
                 for (int q = 0; q < 4; q++)
-                    acc[i][j][q] = Cp[(int64_t)col * NB + r_base + wm * 64 + i * 16 + MF<T>::row(q, fk0)];
+                    acc[i][j][q] = Cp[(int64_t)(c_base + wn * 64 + j * 16 + MF<T>::row(q, fk0)) * NB + row];
             }
     }
     const int nstage = (NB + BK - 1) / BK;
@@ -700,28 +681,27 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    acc[i][j] = MF<T>::mfma(a[i], b[j], acc[i][j]);
+                    acc[i][j] = MF<T>::mfma(b[j], a[i], acc[i][j]);   // C^T: lanes walk rows
         }
         if (st + 1 < nstage) lstore(buf ^ 1);
         __syncthreads();
     }
 
-    // epilogue: element q of lane l holds row 4*q + l/16, column l%16 of each
-    // 16x16 tile (v_mfma_f64_16x16x4_f64 output layout, probed on gfx950)
+    // epilogue: the MFMA was fed (B, A), so element q of lane l holds row
+    // l % 16 and column row(q, l / 16) of each 16x16 tile: a store walks rows
     const bool diag = (m == n) && (bm == bn);
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 4; i++) {
+        const int rr = r_base + wm * 64 + i * 16 + fr;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int col = c_base + wn * 64 + j * 16 + fr;
-            if (col >= cols_n) continue;
+        for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int rr = r_base + wm * 64 + i * 16 + MF<T>::row(q, fk);
+                const int col = c_base + wn * 64 + j * 16 + MF<T>::row(q, fk);
                 if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
-                else if (rr < rows_m && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
+                else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
             }
-        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -777,10 +757,10 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int col = c_base + wn * 64 + j * 16 + fr;
+                const int row = r_base + wm * 64 + i * 16 + fr;
 #pragma unroll
                 for (int q = 0; q < 4; q++)
-                    acc[i][j][q] = Cp[(int64_t)col * NB + r_base + wm * 64 + i * 16 + 4 * fk + q];
+                    acc[i][j][q] = Cp[(int64_t)(c_base + wn * 64 + j * 16 + 4 * fk + q) * NB + row];
             }
     }
 
@@ -850,27 +830,27 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                // fed (B, A): C^T, so that lanes walk rows in the epilogue
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
             }
         __syncthreads();
     }
 
     const bool diag = (m == n) && (bm == bn);
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 4; i++) {
+        const int rr = r_base + wm * 64 + i * 16 + fr;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int col = c_base + wn * 64 + j * 16 + fr;
-            if (col >= cols_n) continue;
+        for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int rr = r_base + wm * 64 + i * 16 + 4 * fk + q;
+                const int col = c_base + wn * 64 + j * 16 + 4 * fk + q;
                 if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
-                else if (rr < rows_m && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
+                else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
             }
-        }
+    }
 }
 
 // ---------------------------------------------------------------------------

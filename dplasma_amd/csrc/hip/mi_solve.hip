@@ -202,6 +202,112 @@ __global__ __launch_bounds__(TSV_THREADS) void k_trsv(const T *__restrict__ Tt, 
     }
 }
 
+// Diagonal solve of tile k for 2..16 right-hand sides (NB <= 512) on MFMA.
+// One block of 512 threads; wave w keeps y for the 16-row tiles g = w + 8t
+// (t < 4) in MFMA accumulator layout (element q of lane l = y(16g +
+// row(q, l/16), l % 16), the right-hand side in l % 16).  Per 32-row block:
+//   the owners of its two row tiles publish y, W = inv(L_cb,cb) is staged,
+//   x_cb = W y_cb (W^T y_cb backward) by all threads from LDS, written to B,
+//   every row tile strictly past the block (forward: below, backward: above)
+//   accumulates -L_strip x_cb with 8 MFMAs (16 rows x 32 x 16 right-hand sides),
+//   its strip of L loaded while the block was published and solved.
+// Forward: strip = L(rows, c0..c0+31); backward: strip = L(c0..c0+31, rows)^T.
+template <class T, bool TRANS>
+__global__ __launch_bounds__(TSV_THREADS) void k_trsv_mf(const T *__restrict__ Tt, int N, int NB, int MT, int k,
+                                                         const T *__restrict__ W, T *__restrict__ B, int ldb,
+                                                         int nr)
+{
+    typedef typename MF<T>::v4 V4;
+    __shared__ T Ys[32][17];
+    __shared__ T Xb[32][17];
+    __shared__ T Ws[32][33];                          // Ws[i][q] = W(i, q)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int fr = lane & 15, fk = lane >> 4;
+    const int n = tile_rows(N, NB, MT, k);
+    const int nblk = (NB + 31) / 32;
+    const int ncb = (n + 31) / 32;
+    const T *L = Tt + tile_index(MT, k, k) * (int64_t)NB * NB;
+    const T *Wk = W + (int64_t)k * nblk * 1024;
+    T *Bk = B + (int64_t)k * NB;
+
+    V4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int g = wave + 8 * t;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = 16 * g + MF<T>::row(q, fk);
+            acc[t][q] = (r < n && fr < nr) ? Bk[(int64_t)fr * ldb + r] : T(0);
+        }
+    }
+    // strip operands of the row tiles this wave updates after block cb:
+    // A(i, kk) = -L(16g + i, c0 + kk) forward, -L(c0 + kk, 16g + i) backward
+    T sa[4][8];
+    auto load_strip = [&](int cb) {
+        const int c0 = cb * 32;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int g = wave + 8 * t;
+            const int r = 16 * g + fr;
+            const bool on = TRANS ? (16 * g + 16 <= c0) : (16 * g >= c0 + 32);
+            const int rc = min(r, NB - 1);
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                const int kk = c0 + 4 * s + fk;
+                const int kc = min(kk, NB - 1);
+                const T v = TRANS ? L[(int64_t)rc * NB + kc] : L[(int64_t)kc * NB + rc];
+                sa[t][s] = (on && r < n && kk < n) ? -v : T(0);
+            }
+        }
+    };
+
+    int cb = TRANS ? ncb - 1 : 0;
+    load_strip(cb);
+    T w0 = Wk[(int64_t)cb * 1024 + tid], w1 = Wk[(int64_t)cb * 1024 + tid + TSV_THREADS];
+    for (int it = 0; it < ncb; it++) {
+        const int c0 = cb * 32;
+        const int nx = TRANS ? cb - 1 : cb + 1;
+        // (a) owners publish y of this block's two row tiles; W to LDS
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int g = wave + 8 * t;
+            if (g == 2 * cb || g == 2 * cb + 1) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) Ys[16 * (g - 2 * cb) + MF<T>::row(q, fk)][fr] = acc[t][q];
+            }
+        }
+        Ws[tid & 31][tid >> 5] = w0;
+        Ws[tid & 31][(tid >> 5) + 16] = w1;
+        __syncthreads();
+        if (it + 1 < ncb) {
+            w0 = Wk[(int64_t)nx * 1024 + tid];
+            w1 = Wk[(int64_t)nx * 1024 + tid + TSV_THREADS];
+        }
+        // (c) x = W y (forward) or W^T y (backward): thread = (row i, rhs j)
+        {
+            const int i = tid >> 4, j = tid & 15;
+            T x = T(0);
+#pragma unroll
+            for (int q = 0; q < 32; q++) x += (TRANS ? Ws[q][i] : Ws[i][q]) * Ys[q][j];
+            Xb[i][j] = x;
+            if (c0 + i < n && j < nr) Bk[(int64_t)j * ldb + c0 + i] = x;
+        }
+        __syncthreads();
+        // (d) row tiles past the block: acc -= L_strip x_cb
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int g = wave + 8 * t;
+            const bool on = TRANS ? (16 * g + 16 <= c0) : (16 * g >= c0 + 32);
+            if (on && 16 * g < n) {
+#pragma unroll
+                for (int s = 0; s < 8; s++) acc[t] = MF<T>::mfma(sa[t][s], Xb[4 * s + fk][fr], acc[t]);
+            }
+        }
+        if (it + 1 < ncb) load_strip(nx);
+        cb = nx;
+    }
+}
+
 // Forward update B_m -= L_mk X_k for m = k+1 .. MT-1.
 // Grid = (MT-k-1) * ceil(NB/64) strips of 64 rows; 4 waves split the NB
 // columns of L_mk, lane = row, 16 independent loads in flight per lane;
@@ -415,9 +521,11 @@ int solve_chunk(Ctx *c, hipStream_t s, int trans, int N, int NB, int MT, const T
         for (int k = 0; k < MT; k++) {
             const int n = tile_rows(N, NB, MT, k);
             MI_LAUNCH(c, s, SN<T>::diag, (double)n * n * nr, (double)n * n * sizeof(T) / 2.0,
-                      (NB <= TSV_THREADS
-                       ? k_trsv<T, NR, false, 1><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)
-                       : k_trsv<T, NR, false, RPT2><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)));
+                      (NB > TSV_THREADS
+                       ? k_trsv<T, NR, false, RPT2><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)
+                       : NR > 1
+                       ? k_trsv_mf<T, false><<<1, TSV_THREADS, 0, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)
+                       : k_trsv<T, NR, false, 1><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)));
             if (k + 1 < MT) {
                 const double rows = (double)(N - (k + 1) * NB);
                 MI_LAUNCH(c, s, SN<T>::gemv, 2.0 * rows * NB * nr, rows * NB * sizeof(T),
@@ -429,9 +537,11 @@ int solve_chunk(Ctx *c, hipStream_t s, int trans, int N, int NB, int MT, const T
         for (int k = MT - 1; k >= 0; k--) {
             const int n = tile_rows(N, NB, MT, k);
             MI_LAUNCH(c, s, SN<T>::diag, (double)n * n * nr, (double)n * n * sizeof(T) / 2.0,
-                      (NB <= TSV_THREADS
-                       ? k_trsv<T, NR, true, 1><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)
-                       : k_trsv<T, NR, true, RPT2><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)));
+                      (NB > TSV_THREADS
+                       ? k_trsv<T, NR, true, RPT2><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)
+                       : NR > 1
+                       ? k_trsv_mf<T, true><<<1, TSV_THREADS, 0, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)
+                       : k_trsv<T, NR, true, 1><<<1, TSV_THREADS, lds_y, s>>>(dT, N, NB, MT, k, W, Bj, ldb, nr)));
             if (k > 0) {
                 const double cols = (double)k * NB;
                 MI_LAUNCH(c, s, SN<T>::gemv, 2.0 * cols * n * nr, cols * n * sizeof(T),

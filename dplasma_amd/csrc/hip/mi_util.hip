@@ -439,9 +439,8 @@ int mi_dcheck_axmb(mi_ctx_t *c, int N, int NB, const double *dA0, int NRHS, cons
     hipStream_t s = c->s_main;
     k_zero<<<grid_for(npad * NRHS + npad), 256, 0, s>>>(buf, npad * NRHS + npad);
     int rc;
-    if ((rc = dsymm_resid(c, s, N, NB, dA0, NRHS, dx, ldx, db, ldb, r, (int)npad))) goto out;
-    k_sym_rowsum<<<grid_for((int64_t)MT * (MT + 1) / 2 * NB), 256, 0, s>>>(dA0, N, NB, MT, rs);
-    if ((rc = dmax_of(c, rs, npad, Anorm))) goto out;
+    if ((rc = dsymm_resid(c, s, N, NB, dA0, NRHS, dx, ldx, db, ldb, r, (int)npad, rs))) goto out;
+    if ((rc = dmax_of(c, rs, N, Anorm))) goto out;       // row sums of |A0| from the residual pass
     k_ge_rowsum<<<grid_for(N), 256, 0, s>>>(db, N, NRHS, ldb, rs);
     if ((rc = dmax_of(c, rs, N, Bnorm))) goto out;
     k_ge_rowsum<<<grid_for(N), 256, 0, s>>>(dx, N, NRHS, ldx, rs);
