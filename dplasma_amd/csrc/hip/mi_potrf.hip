@@ -326,6 +326,62 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
     }
 }
 
+// TRSM(m,k) for few panel tiles (latency-bound steps): one workgroup per
+// 16-row strip; the left-looking product X[:, 0:j0] L[j0:j0+32, 0:j0]^T of
+// each 32-column block is split over the 4 waves by K (contiguous 32-column
+// slices, partial sums met in LDS), so the per-block chain is a quarter of
+// the 64-row kernel's; then P = C - sum, X = P W_jj^T by all threads.
+constexpr int TSK_ROWS = 16;
+template <class T>
+__global__ __launch_bounds__(256) void k_trsm_tile_ks(T *__restrict__ Tt, int N, int NB, int MT, int k,
+                                                     const T *__restrict__ Winv)
+{
+    __shared__ T red[4][32][TSK_ROWS + 1];       // partial products [wave][col][row]
+    __shared__ T P[32][TSK_ROWS + 1];
+    __shared__ T Ws[1024];
+    typedef typename MF<T>::v4 V4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int fr = lane & 15, fk = lane >> 4;
+    const int strips = (NB + TSK_ROWS - 1) / TSK_ROWS;
+    const int m = k + 1 + blockIdx.x / strips;
+    const int r0 = (blockIdx.x % strips) * TSK_ROWS;
+    const int rows_m = tile_rows(N, NB, MT, m);
+    if (r0 >= rows_m) return;
+    const int rv = min(TSK_ROWS, rows_m - r0);
+    T *C = Tt + tile_index(MT, m, k) * (int64_t)NB * NB + r0;
+    const T *L = Tt + tile_index(MT, k, k) * (int64_t)NB * NB;
+
+    for (int j0 = 0; j0 < NB; j0 += 32) {
+        const int jw = min(32, NB - j0);
+        // this wave's K slice of the j0 solved columns
+        const int np = j0 / 32;
+        const int p0 = (np * wave) / 4, p1 = (np * (wave + 1)) / 4;
+        V4 acc[2] = {{T(0), T(0), T(0), T(0)}, {T(0), T(0), T(0), T(0)}};
+        if (p1 > p0)
+            mfma_panel_2(C + (int64_t)(32 * p0) * NB, NB, 0, rv, L + (int64_t)(32 * p0) * NB + j0, NB, 0, jw,
+                         32 * (p1 - p0), acc);
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) red[wave][ct * 16 + fr][MF<T>::row(q, fk)] = acc[ct][q];
+        for (int idx = tid; idx < 1024; idx += 256) Ws[idx] = Winv[(j0 >> 5) * 1024 + idx];
+        __syncthreads();
+        for (int idx = tid; idx < 32 * TSK_ROWS; idx += 256) {
+            const int r = idx % TSK_ROWS, c = idx / TSK_ROWS;
+            const T sum = red[0][c][r] + red[1][c][r] + red[2][c][r] + red[3][c][r];
+            P[c][r] = (r < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + r] - sum : T(0);
+        }
+        __syncthreads();
+        for (int idx = tid; idx < 32 * TSK_ROWS; idx += 256) {
+            const int r = idx % TSK_ROWS, c = idx / TSK_ROWS;
+            T x = T(0);
+            for (int p = 0; p <= c; p++) x += P[p][r] * Ws[p * 32 + c];
+            if (r < rv && c < jw) C[(int64_t)(j0 + c) * NB + r] = x;
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // POTRF(k): factor the n x n diagonal tile A (ld) in place, lower.
 // LDS holds the current panel P (rows j0..n-1, PJB columns) column-major with
@@ -542,8 +598,9 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double *__restrict__ T, int 
 
 // ---------------------------------------------------------------------------
 // Trailing update on MFMA: C(m,n) -= A(m,k) A(n,k)^T for tile columns n in
-// [n_begin, n_end), rows m in [n, MT).  One workgroup = one 128x128 block of
-// one tile; four waves, each a 64x64 quadrant as 4x4 MFMA 16x16 tiles.
+// [n_begin, n_end), rows m in [n, MT).  One workgroup = one BMX x BMX block of
+// one tile (BMX = 128, or 64 for launches too small to fill the chip); four
+// waves, each a BMX/2 quadrant as (BMX/32)^2 MFMA 16x16 tiles.
 constexpr int BM = 128, BK = 16;
 
 // workgroup L of an update launch -> tile (m, n) and its 128x128 block
@@ -567,14 +624,16 @@ __device__ inline void decode_block(int L, int nsub, int n_begin, int m_off, int
 // The FP64 MFMA pipe of a gfx950 SIMD needs two resident waves to stay
 // busy (one wave alone issues at half rate), so the kernel is held to 256
 // registers per lane (128 accumulators) and two workgroups share each CU.
-template <class T>
+template <class T, int BMX>
 __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int N, int NB, int MT, int k,
                                                        int n_begin, int m_off, int nsub)
 {
-    __shared__ T As[2][BK][BM];
-    __shared__ T Bs[2][BK][BM];
+    __shared__ T As[2][BK][BMX];
+    __shared__ T Bs[2][BK][BMX];
     typedef typename MF<T>::v4 V4;
     typedef typename MF<T>::v2 V2;
+    constexpr int TI = BMX / 32;                      // 16x16 MFMA tiles per wave and dimension
+    constexpr int HB = BMX / 2;                       // rows (columns) per wave
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
 
@@ -591,37 +650,37 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
 
     const int rows_m = tile_rows(N, NB, MT, m);
     const int cols_n = tile_rows(N, NB, MT, n);
-    const int r_base = bm * BM, c_base = bn * BM;
+    const int r_base = bm * BMX, c_base = bn * BMX;
     if (r_base >= rows_m || c_base >= cols_n) return;
-    const bool full = (r_base + BM <= rows_m) && (c_base + BM <= cols_n) && (NB % BK == 0) && !(NB & 1);
+    const bool full = (r_base + BMX <= rows_m) && (c_base + BMX <= cols_n) && (NB % BK == 0) && !(NB & 1);
 
     const T *Ap = Tt + tile_index(MT, m, k) * (int64_t)NB * NB;
     const T *Bp = Tt + tile_index(MT, n, k) * (int64_t)NB * NB;
     T *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
 
-    V4 acc[4][4];
+    V4 acc[TI][TI];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < TI; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = V4{T(0), T(0), T(0), T(0)};
+        for (int j = 0; j < TI; j++) acc[i][j] = V4{T(0), T(0), T(0), T(0)};
 
-    // global -> register staging in 16-byte pairs: pair e = tid + 256*i,
-    // k row kk = e / 64, rows 2*(e % 64) and 2*(e % 64) + 1
-    V2 ra[4], rb[4];
+    // global -> register staging in element pairs: pair e = tid + 256*i,
+    // k row kk = e / HB, rows 2*(e % HB) and 2*(e % HB) + 1
+    V2 ra[TI], rb[TI];
     auto gload = [&](int k0) {
         if (full) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+            for (int i = 0; i < TI; i++) {
                 const int e = tid + 256 * i;
-                const int kk = e >> 6, r = (e & 63) * 2;
+                const int kk = e / HB, r = (e % HB) * 2;
                 ra[i] = *(const V2 *)(Ap + (int64_t)(k0 + kk) * NB + r_base + r);
                 rb[i] = *(const V2 *)(Bp + (int64_t)(k0 + kk) * NB + c_base + r);
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+            for (int i = 0; i < TI; i++) {
                 const int e = tid + 256 * i;
-                const int kk = e >> 6, r = (e & 63) * 2;
+                const int kk = e / HB, r = (e % HB) * 2;
                 const int kg = k0 + kk;
                 T a0 = T(0), a1 = T(0), b0 = T(0), b1 = T(0);
                 if (kg < NB) {
@@ -639,9 +698,9 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < TI; i++) {
             const int e = tid + 256 * i;
-            const int kk = e >> 6, r = (e & 63) * 2;
+            const int kk = e / HB, r = (e % HB) * 2;
             *(V2 *)&As[buf][kk][r] = V2{-ra[i].x, -ra[i].y};     // -A: the MFMA accumulates C - A B^T
             *(V2 *)&Bs[buf][kk][r] = rb[i];
         }
@@ -653,13 +712,13 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
     const bool pre = full && !((m == n) && (bm == bn));
     if (pre) {
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < TI; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int row = r_base + wm * 64 + i * 16 + fr0;
+            for (int j = 0; j < TI; j++) {
+                const int row = r_base + wm * HB + i * 16 + fr0;
 #pragma unroll
                 for (int q = 0; q < 4; q++)
-                    acc[i][j][q] = Cp[(int64_t)(c_base + wn * 64 + j * 16 + MF<T>::row(q, fk0)) * NB + row];
+                    acc[i][j][q] = Cp[(int64_t)(c_base + wn * HB + j * 16 + MF<T>::row(q, fk0)) * NB + row];
             }
     }
     const int nstage = (NB + BK - 1) / BK;
@@ -672,15 +731,15 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
         if (st + 1 < nstage) gload((st + 1) * BK);
 #pragma unroll
         for (int kq = 0; kq < BK / 4; kq++) {
-            T a[4], b[4];
+            T a[TI], b[TI];
 #pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[buf][kq * 4 + fk][wm * 64 + i * 16 + fr];
+            for (int i = 0; i < TI; i++) a[i] = As[buf][kq * 4 + fk][wm * HB + i * 16 + fr];
 #pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = Bs[buf][kq * 4 + fk][wn * 64 + j * 16 + fr];
+            for (int j = 0; j < TI; j++) b[j] = Bs[buf][kq * 4 + fk][wn * HB + j * 16 + fr];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < TI; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++)
+                for (int j = 0; j < TI; j++)
                     acc[i][j] = MF<T>::mfma(b[j], a[i], acc[i][j]);   // C^T: lanes walk rows
         }
         if (st + 1 < nstage) lstore(buf ^ 1);
@@ -691,13 +750,13 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
     // l % 16 and column row(q, l / 16) of each 16x16 tile: a store walks rows
     const bool diag = (m == n) && (bm == bn);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int rr = r_base + wm * 64 + i * 16 + fr;
+    for (int i = 0; i < TI; i++) {
+        const int rr = r_base + wm * HB + i * 16 + fr;
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < TI; j++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int col = c_base + wn * 64 + j * 16 + MF<T>::row(q, fk);
+                const int col = c_base + wn * HB + j * 16 + MF<T>::row(q, fk);
                 if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
                 else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
             }
@@ -1056,8 +1115,15 @@ int launch_trsm(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, cons
     for (int m = k + 1; m < MT; m++) fl += (double)tile_rows(N, NB, MT, m) * NB * NB;
     if (!valu_panel<T>()) {
         const int strips = (NB + TS_ROWS - 1) / TS_ROWS;
-        MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
-                  k_trsm_tile<T><<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+        if (ntile * strips >= 256) {
+            MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
+                      k_trsm_tile<T><<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+        } else {
+            // few tiles: 16-row strips with the K range split over the waves
+            const int ks = (NB + TSK_ROWS - 1) / TSK_ROWS;
+            MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
+                      k_trsm_tile_ks<T><<<ntile * ks, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+        }
     } else {
         const int strips = (NB + TRSM_ROWS - 1) / TRSM_ROWS;
         MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
@@ -1079,8 +1145,21 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
     for (int n = n_begin + 1; n < MT; n++) avail += MT - n;
     if (ntiles < 0 || ntiles > avail) ntiles = avail;
     if (ntiles <= 0) return 0;
-    const int nsub = (NB + BM - 1) / BM;
-    const int64_t grid = ntiles * nsub * nsub;
+    int nsub = (NB + BM - 1) / BM;
+    int64_t grid = ntiles * nsub * nsub;
+    // a launch of fewer than two 128x128 blocks per CU is latency-bound:
+    // 64x64 blocks give four times the workgroups at a quarter of the chain
+    if (g_update_kernel < 0) {
+        const char *e = getenv("DPLASMA_MI_UPDATE");
+        g_update_kernel = (e && !strcmp(e, "valu")) ? 0 : 1;
+    }
+    const bool valu = std::is_same<T, double>::value && g_update_kernel == 0;
+    const bool bf16 = std::is_same<T, float>::value && g_supdate == 1;
+    const bool small = grid < 512 && !valu && !bf16;
+    if (small) {
+        nsub = (NB + 63) / 64;
+        grid = ntiles * nsub * nsub;
+    }
     if (grid > 0x7fffffff) { set_error("update grid too large"); return -3; }
     // algorithmic flops / bytes of the covered tiles
     double fl = 0.0, by = 0.0;
@@ -1091,21 +1170,20 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
             fl += (m == n) ? (double)NB * tm * (tm + 1.0) : 2.0 * tm * NB * NB;
             by += (2.0 * tm * NB + NB * NB) * sizeof(T);
         }
-    if (g_update_kernel < 0) {
-        const char *e = getenv("DPLASMA_MI_UPDATE");
-        g_update_kernel = (e && !strcmp(e, "valu")) ? 0 : 1;
-    }
-    if (std::is_same<T, double>::value && g_update_kernel == 0)
+    if (valu)
         MI_LAUNCH(c, s, "dpotrf_update_valu", fl, by,
                   k_update_valu<<<(unsigned)grid, 256, 0, s>>>((double *)dT, N, NB, MT, k, n_begin, m_off,
                                                                nsub));
-    else if (std::is_same<T, float>::value && g_supdate == 1)
+    else if (bf16)
         MI_LAUNCH(c, s, KN<float>::update_bf16, fl, by,
                   k_update_bf16x3<<<(unsigned)grid, 256, 0, s>>>((float *)dT, N, NB, MT, k, n_begin, m_off,
                                                                  nsub));
+    else if (small)
+        MI_LAUNCH(c, s, KN<T>::update, fl, by,
+                  k_update_mfma<T, 64><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
     else
         MI_LAUNCH(c, s, KN<T>::update, fl, by,
-                  k_update_mfma<T><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
+                  k_update_mfma<T, BM><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
     return 0;
 }
 

@@ -227,3 +227,25 @@ def test_dposv_mixed_not_spd(ctx, dp, po):
     info, it = ctx.dposv_mixed(dp.LOWER, A, B, X)
     assert info > 0 and it == -3, (info, it)
     A.free(); B.free(); X.free()
+
+
+@pytest.mark.parametrize("uplo", ["L"])
+def test_dpotrf_large_resident_gpu_checks(ctx, dp, uplo):
+    """A size whose early panels take the 64-row TRSM kernel and whose late
+    panels take the K-split one (>= 32 panel tiles at step 0): the
+    reference's check_dpotrf and check_daxmb, computed on the GPU."""
+    N, NB = 16896, 512
+    u = dp.LOWER
+    A = dp.SymMatrix(ctx, N, NB, u)
+    A0 = dp.SymMatrix(ctx, N, NB, u)
+    for M in (A, A0):
+        assert ctx.advise(u, M) == 0
+        assert ctx.dplgsy(float(N), u, M, 3872) == 0
+    assert ctx.dpotrf(u, A) == 0
+    assert ctx.check_dpotrf(0, u, A, A0) == 0
+    B = dp.GeMatrix(ctx, N, 3, NB, NB); ctx.dplrnt(0, B, 3873)
+    X = dp.GeMatrix(ctx, N, 3, NB, NB); ctx.dplrnt(0, X, 3873)
+    assert ctx.dpotrs(u, A, X) == 0
+    assert ctx.check_daxmb(0, u, A0, B, X) == 0
+    for M in (A, A0, B, X):
+        M.free()

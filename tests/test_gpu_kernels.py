@@ -110,8 +110,10 @@ def test_panel_trsm(dev, po, N, NB, panel):
 
 
 @pytest.mark.parametrize("engine", [0, 1], ids=["valu", "mfma"])
-@pytest.mark.parametrize("N,NB", [(378, 93), (1024, 256), (1536, 512), (700, 128), (1000, 320)])
+@pytest.mark.parametrize("N,NB", [(378, 93), (1024, 256), (1536, 512), (700, 128), (1000, 320), (5120, 512)])
 def test_trailing_update(dev, po, N, NB, engine):
+    """One step's trailing update; small launches run 64x64 blocks, the
+    5120/512 case (720 blocks of 128x128) the 128x128 kernel."""
     rng = np.random.default_rng(N)
     MT = (N + NB - 1) // NB
     T = rng.standard_normal(n_sym(N, NB))
