@@ -628,8 +628,11 @@ template <class T, int BMX>
 __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int N, int NB, int MT, int k,
                                                        int n_begin, int m_off, int nsub)
 {
-    __shared__ T As[2][BK][BMX];
-    __shared__ T Bs[2][BK][BMX];
+    // rows padded by 128 bytes: the four lane groups of a fragment read (k
+    // rows 4s + l/16) then fall on alternating halves of the LDS banks
+    constexpr int LP = BMX + 128 / (int)sizeof(T);
+    __shared__ T As[2][BK][LP];
+    __shared__ T Bs[2][BK][LP];
     typedef typename MF<T>::v4 V4;
     typedef typename MF<T>::v2 V2;
     constexpr int TI = BMX / 32;                      // 16x16 MFMA tiles per wave and dimension
