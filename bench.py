@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TF = 78.6          # MI355X dense FP64 matrix peak, TFLOP/s
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, GB/s
 UPDATE_KERNEL = "dpotrf_update"   # trailing SYRK/GEMM update, the dominant kernel
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dpotrf_update_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dpotrf_update_N32768_NB512_traffic.json")
 
 
 def flops_dpotrf(n):
