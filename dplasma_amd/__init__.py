@@ -126,6 +126,8 @@ def libs():
     _sig(M, "mi_spotrf_step_update", i, [vp, i, i, vp, i, i, i])
     _sig(M, "mi_strsm_LL", i, [vp, i, i, ctypes.c_float, i, i, vp, i, vp, i])
     _sig(M, "mi_spotrs_tiles", i, [vp, i, i, vp, i, vp, i])
+    _sig(M, "mi_dpotrf_adjoint_tiles", i, [vp, i, i, vp, vp])
+    _sig(M, "mi_spotrf_adjoint_tiles", i, [vp, i, i, vp, vp])
     _sig(M, "mi_dpotrf_step_potrf", i, [vp, i, i, vp, i, pi])
     _sig(M, "mi_dpotrf_step_trsm", i, [vp, i, i, vp, i])
     _sig(M, "mi_dpotrf_step_update", i, [vp, i, i, vp, i, i, i])

@@ -164,6 +164,19 @@ int mi_dsymm_resid(mi_ctx_t *ctx, int N, int NB, const double *dA, int NRHS, con
  * FLT_MAX (LAPACK dlag2s).  Blocking. */
 int mi_dlag2s_tiles(mi_ctx_t *ctx, int64_t n, const double *dS, float *dD, int *overflow);
 
+/* ------------------------------------------------------------ differentiation */
+/* Reverse-mode derivative of the tile Cholesky (no counterpart in the
+ * reference: the reverse of its right-looking factorization,
+ * src/zpotrf_L.jdf:91-633).  dL: the factor L of A = L L^T as returned by
+ * mi_dpotrf_tiles (lower tiles; strict upper parts of the diagonal tiles and
+ * padding rows are not read).  dLbar: on entry the gradient of a scalar f
+ * with respect to L (lower tiles, same layout); on exit the symmetric
+ * gradient Abar with respect to A (df = sum_ij Abar_ij dA_ij, lower tiles;
+ * the strict upper parts of diagonal tiles and padding are not written).
+ * Blocking; returns 0 or a negative error. */
+int mi_dpotrf_adjoint_tiles(mi_ctx_t *ctx, int N, int NB, const double *dL, double *dLbar);
+int mi_spotrf_adjoint_tiles(mi_ctx_t *ctx, int N, int NB, const float *dL, float *dLbar);
+
 /* ------------------------------------------------------------ checks */
 /* Norms and residuals behind check_dpotrf / check_daxmb
  * (src/dplasma_zcheck.c:63-133,179-224) computed on the GPU:
