@@ -166,6 +166,7 @@ def libs():
     _sig(D, "dplasma_dposv", i, [vp, i, vp, vp])
     _sig(D, "dplasma_dsposv", i, [vp, i, vp, vp, vp, pi])
     _sig(D, "dplasma_dposv_mixed", i, [vp, i, vp, vp, vp, i, pi])
+    _sig(D, "dplasma_dpotrf_adjoint", i, [vp, i, vp, vp])
     _sig(D, "dplasma_dtrsm", i, [vp, i, i, i, i, d, vp, vp])
     _sig(D, "dplasma_dtrsm_New", vp, [i, i, i, i, d, vp, vp])
     _sig(D, "dplasma_dtrsm_Destruct", None, [vp])
@@ -331,6 +332,10 @@ class Context:
         it = ctypes.c_int(0)
         info = self.D.dplasma_dposv_mixed(self.ptr, uplo, A.ptr, B.ptr, X.ptr, prec, ctypes.byref(it))
         return info, it.value
+
+    def dpotrf_adjoint(self, uplo, L, Lbar):
+        """Lbar (df/dL) -> symmetric df/dA in place; returns 0 or an error code."""
+        return self.D.dplasma_dpotrf_adjoint(self.ptr, uplo, L.ptr, Lbar.ptr)
 
     def dtrsm(self, side, uplo, trans, diag, alpha, A, B):
         return self.D.dplasma_dtrsm(self.ptr, side, uplo, trans, diag, float(alpha), A.ptr, B.ptr)

@@ -299,3 +299,35 @@ int dplasma_dsposv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_m
 {
     return dplasma_dposv_mixed(parsec, uplo, A, B, X, DPLASMA_MIXED_FP32, iter);
 }
+
+/* Reverse-mode derivative of dplasma_dpotrf (extension, no counterpart in
+ * the reference): L is the factor returned by dplasma_dpotrf (uplo part),
+ * Lbar holds df/dL on entry (same uplo part) and the symmetric df/dA on
+ * exit.  An Upper descriptor holds U = L^T and df/dU = (df/dL)^T: its lower
+ * image is the lower problem, and the symmetric result maps back unchanged. */
+int dplasma_dpotrf_adjoint(parsec_context_t *parsec, dplasma_enum_t uplo, const parsec_tiled_matrix_t *L,
+                           parsec_tiled_matrix_t *Lbar)
+{
+    if (uplo != dplasmaUpper && uplo != dplasmaLower) {
+        dpl_error("dplasma_dpotrf_adjoint", "illegal value of uplo");
+        return -1;
+    }
+    if (L->m != Lbar->m || L->n != Lbar->n || L->mb != Lbar->mb || L->nb != Lbar->nb) {
+        dpl_error("dplasma_dpotrf_adjoint", "L and Lbar must have the same shape and tiling");
+        return -2;
+    }
+    dpl_sym_dev_t sl, sg;
+    memset(&parsec->last, 0, sizeof parsec->last);
+    int rc = dpl_sym_acquire(parsec, L, uplo, 1, &sl, &parsec->last.stage_in_ms);
+    if (rc) return rc;
+    if ((rc = dpl_sym_acquire(parsec, Lbar, uplo, 1, &sg, &parsec->last.stage_in_ms))) {
+        dpl_sym_release(parsec, L, &sl, 0, NULL);
+        return rc;
+    }
+    rc = mi_dpotrf_adjoint_tiles(parsec->mi, sl.N, sl.NB, sl.d, sg.d);
+    if (rc) rc = dpl_mi_fail("dplasma_dpotrf_adjoint", rc);
+    else mi_last_span_ms(parsec->mi, &parsec->last.total_ms);
+    int r2 = dpl_sym_release(parsec, Lbar, &sg, rc == 0, &parsec->last.stage_out_ms);
+    dpl_sym_release(parsec, L, &sl, 0, NULL);
+    return rc ? rc : r2;
+}

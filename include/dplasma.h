@@ -71,6 +71,14 @@ int dplasma_dsposv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_m
 int dplasma_dposv_mixed(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A,
                         parsec_tiled_matrix_t *B, parsec_tiled_matrix_t *X, int prec, int *iter);
 
+/* ---------------- differentiation (extension: the reverse of dplasma_dpotrf,
+ * src/zpotrf_L.jdf:91-633; no counterpart in the reference).  L: the factor
+ * dplasma_dpotrf returned (uplo part); Lbar: df/dL on entry (uplo part, the
+ * transpose convention of U = L^T for Upper), the symmetric df/dA on exit
+ * (uplo part; df = sum_ij Abar_ij dA_ij).  Returns 0, -1 (uplo), -2 (shape). */
+int dplasma_dpotrf_adjoint(parsec_context_t *parsec, dplasma_enum_t uplo, const parsec_tiled_matrix_t *L,
+                           parsec_tiled_matrix_t *Lbar);
+
 /* ---------------- triangular solve (dplasma_z.h:51,153,221) */
 int dplasma_dtrsm(parsec_context_t *parsec, dplasma_enum_t side, dplasma_enum_t uplo,
                   dplasma_enum_t trans, dplasma_enum_t diag, double alpha,

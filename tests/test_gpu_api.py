@@ -6,6 +6,8 @@ sizes the reference's ctest registers (tests/Testings.cmake:98-99,164-167,
 Tolerances: FP64 factors agree with the oracle's tile-DAG factor to a
 relative 1e-12; residual ratios are below the reference threshold 60.
 """
+import os
+import sys
 import numpy as np
 import pytest
 
@@ -249,3 +251,36 @@ def test_dpotrf_large_resident_gpu_checks(ctx, dp, uplo):
     assert ctx.check_daxmb(0, u, A0, B, X) == 0
     for M in (A, A0, B, X):
         M.free()
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_dpotrf_adjoint_descriptors(ctx, dp, po, uplo):
+    """dplasma_dpotrf_adjoint on descriptors: factor with dplasma_dpotrf, then
+    the symmetric gradient of f = sum(W * L) against the oracle's closed form
+    (Upper descriptors hold U = L^T and df/dU = W^T)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import adjoint as oad
+    N, NB = 1000, 192
+    u = dp.LOWER if uplo == "L" else dp.UPPER
+    pu = po.LOWER if uplo == "L" else po.UPPER
+    A = dp.SymMatrix(ctx, N, NB, u); ctx.dplgsy(float(N), u, A, 3872)
+    assert ctx.dpotrf(u, A) == 0
+    S = po.symtiles_to_full(A.array, N, NB, pu)
+    L = np.tril(S) if uplo == "L" else np.triu(S).T
+    W = np.tril(np.random.default_rng(4).standard_normal((N, N)))
+    G = dp.SymMatrix(ctx, N, NB, u)
+    G.array[:] = po.full_to_symtiles(W if uplo == "L" else W.T, NB, pu)
+    assert ctx.dpotrf_adjoint(u, A, G) == 0
+    R = po.symtiles_to_full(G.array, N, NB, pu)
+    got = np.tril(R) if uplo == "L" else np.triu(R).T
+    want = np.tril(oad.adjoint_closed(L, W))
+    assert np.abs(got - want).max() <= 1e-12 * N * np.abs(want).max()
+    A.free(); G.free()
+
+
+def test_dpotrf_adjoint_bad_args(ctx, dp):
+    A = dp.SymMatrix(ctx, 100, 32, dp.LOWER)
+    G = dp.SymMatrix(ctx, 120, 32, dp.LOWER)
+    assert ctx.dpotrf_adjoint(dp.UPPERLOWER, A, A) == -1
+    assert ctx.dpotrf_adjoint(dp.LOWER, A, G) == -2
+    A.free(); G.free()
