@@ -76,9 +76,18 @@ template <class T, int BMX>
 __global__ __launch_bounds__(256, 2) void k_gemm_jobs(const GJob<T> *__restrict__ jobs, int nsub, int KS,
                                                      T *__restrict__ part)
 {
+    // LDS layouts.  op(A) with rows contiguous in memory (form 0) is staged
+    // k-major, As[kk * LP + i] (rows padded by 128 bytes: the four lane
+    // groups of a fragment read fall on alternating bank halves).  Operands
+    // contiguous along k (B, and op(A) = A^T) are staged row-major with a
+    // short pad, Xs[c * KR + kk]: one 16-byte store per element pair, and
+    // the 16 rows x 4 k of a fragment read hit distinct banks (KR = 18
+    // doubles, 20 floats).
     constexpr int LP = BMX + 128 / (int)sizeof(T);
-    __shared__ T As[2][GK][LP];
-    __shared__ T Bs[2][GK][LP];
+    constexpr int KR = GK + (sizeof(T) == 8 ? 2 : 4);
+    constexpr int SZ = (GK * LP > BMX * KR) ? GK * LP : BMX * KR;
+    __shared__ T As[2][SZ];
+    __shared__ T Bs[2][SZ];
     typedef typename MF<T>::v4 V4;
     typedef typename MF<T>::v2 V2;
     constexpr int TI = BMX / 32;
@@ -187,22 +196,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm_jobs(const GJob<T> *__restrict_
             }
         }
     };
+    int form0 = 0, form1 = 0;   // layout of the op(A) stage held in buffer 0 / 1
     auto lstore = [&](int buf) {
 #pragma unroll
         for (int p = 0; p < NP; p++) {
             const int e = tid + 256 * p;
             if (st_form == 0) {
                 const int kk = e / HB, r = (e % HB) * 2;
-                *(V2 *)&As[buf][kk][r] = ra[p];
+                *(V2 *)&As[buf][kk * LP + r] = ra[p];
             } else {
                 const int r = e / (GK / 2), kk = (e % (GK / 2)) * 2;
-                As[buf][kk][r] = ra[p].x;
-                As[buf][kk + 1][r] = ra[p].y;
+                *(V2 *)&As[buf][r * KR + kk] = ra[p];
             }
             const int c = e / (GK / 2), kk = (e % (GK / 2)) * 2;
-            Bs[buf][kk][c] = rb[p].x;
-            Bs[buf][kk + 1][c] = rb[p].y;
+            *(V2 *)&Bs[buf][c * KR + kk] = rb[p];
         }
+        if (buf) form1 = st_form;
+        else form0 = st_form;
     };
 
     const int fr = lane & 15, fk = lane >> 4;
@@ -216,10 +226,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_jobs(const GJob<T> *__restrict_
 #pragma unroll
             for (int kq = 0; kq < GK / 4; kq++) {
                 T a[TI], b[TI];
+                const int kk = kq * 4 + fk;
+                if ((buf ? form1 : form0) == 0) {
 #pragma unroll
-                for (int i = 0; i < TI; i++) a[i] = As[buf][kq * 4 + fk][wm * HB + i * 16 + fr];
+                    for (int i = 0; i < TI; i++) a[i] = As[buf][kk * LP + wm * HB + i * 16 + fr];
+                } else {
 #pragma unroll
-                for (int jj = 0; jj < TI; jj++) b[jj] = Bs[buf][kq * 4 + fk][wn * HB + jj * 16 + fr];
+                    for (int i = 0; i < TI; i++) a[i] = As[buf][(wm * HB + i * 16 + fr) * KR + kk];
+                }
+#pragma unroll
+                for (int jj = 0; jj < TI; jj++) b[jj] = Bs[buf][(wn * HB + jj * 16 + fr) * KR + kk];
 #pragma unroll
                 for (int i = 0; i < TI; i++)
 #pragma unroll
@@ -265,11 +281,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_jobs(const GJob<T> *__restrict_
 }
 
 // Sum of the KS partial blocks in split order, then the job's epilogue.
+// RS_SLICE elements of one block per workgroup (8 per thread).
+constexpr int RS_SLICE = 2048;
+
 template <class T, int BMX>
 __global__ __launch_bounds__(256) void k_gemm_reduce(const GJob<T> *__restrict__ jobs, int nsub, int KS,
                                                     const T *__restrict__ part)
 {
-    const int L = blockIdx.x;
+    constexpr int NSL = BMX * BMX / RS_SLICE;
+    const int L = blockIdx.x / NSL, sl = blockIdx.x - (blockIdx.x / NSL) * NSL;
     const int j = L / (nsub * nsub), blk = L - j * nsub * nsub;
     const int bm = blk / nsub, bn = blk - bm * nsub;
     const GJob<T> &J = jobs[j];
@@ -277,7 +297,7 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const GJob<T> *__restrict__
     if (r_base >= J.rows || c_base >= J.cols) return;
     const bool upper_blk = J.outmode >= 1 && r_base + BMX <= c_base;
     const T *P = part + (int64_t)(j * nsub * nsub + blk) * KS * BMX * BMX;
-    for (int e = threadIdx.x; e < BMX * BMX; e += 256) {
+    for (int e = sl * RS_SLICE + threadIdx.x; e < (sl + 1) * RS_SLICE; e += 256) {
         const int rl = e % BMX, cl = e / BMX;
         const int rr = r_base + rl, col = c_base + cl;
         if (rr >= J.rows || col >= J.cols) continue;
@@ -629,7 +649,7 @@ int adjoint_tiles(mi_ctx_t *c, int N, int NB, const T *dL, T *dG)
             else
                 MI_LAUNCH(c, s, AN<T>::gemm, l.flops, 0.0, k_gemm_jobs<T, 64><<<grid, 256, 0, s>>>(jj, l.nsub, l.ks, part));
             if (l.ks > 1) {
-                const unsigned rg = (unsigned)((int64_t)l.njobs * l.nsub * l.nsub);
+                const unsigned rg = (unsigned)((int64_t)l.njobs * l.nsub * l.nsub * l.bmx * l.bmx / RS_SLICE);
                 if (l.bmx == 128)
                     MI_LAUNCH(c, s, AN<T>::reduce, 0.0, 0.0, k_gemm_reduce<T, 128><<<rg, 256, 0, s>>>(jj, l.nsub, l.ks, part));
                 else
