@@ -12,8 +12,11 @@ dplrnt(seed 3873)) with the reference's residual ratios computed on the GPU,
 the mixed-precision dposv at N = 32768, NRHS = 16 on A = Q diag(lambda) Q^T
 for kappa in {1e2, 1e5, 1e8} against the FP64 GPU dpotrf + dpotrs of the same
 system (time, refinement steps, backward error, error vs the FP64 answer),
-the roofline of the dominant kernel (the trailing update, timed with HIP
-events on its own stream in a separate profiled step) and the CPU baseline:
+the adjoint of the factorization (df/dA from a given df/dL) at the same
+size, the differentiated Gaussian-process likelihood (N = 8192, FP64 and
+FP32, plus 64 GPs of 1024 points), the roofline of the dominant kernel (the
+trailing update, timed with HIP events on its own stream in a separate
+profiled step) and the CPU baseline:
 the oracle's tile-DAG executor (a restatement of src/zpotrf_L.jdf with
 OpenBLAS tile kernels) on the host cores, on a bounded sample.
 
@@ -147,6 +150,112 @@ def mixed_leg(dp, N, NB, nrhs=16, kappas=(1e2, 1e5, 1e8)):
     return out
 
 
+def flops_adjoint(N, NB):
+    """Flops of the blocked adjoint as its kernels run them: S1 2 M^2 NB,
+    S2 + S3 4 M NB^2 per step (M = rows below the diagonal tile), and the
+    three NB^3 products of the diagonal tile (6 NB^3) per step."""
+    MT = (N + NB - 1) // NB
+    f = 0.0
+    for k in range(MT):
+        M = max(0, N - (k + 1) * NB)
+        f += 2.0 * M * M * NB + 4.0 * M * NB * NB + 6.0 * min(NB, N - k * NB) ** 3
+    return f
+
+
+def gp_leg(dp, N=8192, NB=512, batch=64, nb_small=1024):
+    """Differentiated Gaussian-process NLL: N points uniform in [0,1]^3
+    (seed 5), SE kernel (lengthscale 0.2, variance 1), noise 1e-2, y = L z
+    ~ N(0, K).  Forward: tile Cholesky, two triangular solves, logdet;
+    backward: the blocked adjoint (torch autograd over the C ABI).  The
+    gradient dNLL/dK is compared with the analytic (K^-1 - alpha alpha^T)/2
+    (torch's own FP64 Cholesky inverse) and dNLL/dl with central differences
+    (h = 1e-6).  Also FP32 (the kernel matrix rounded to FP32) and a batch
+    of 64 GPs of 1024 points each (one after another)."""
+    import ctypes
+    import torch
+    import dplasma_amd.autograd as ag
+    dev = torch.device("cuda")
+    f64 = torch.float64
+
+    def last_ms():
+        d = ag._Lib.get()
+        ms = ctypes.c_double()
+        d.M.mi_last_span_ms(d.h, ctypes.byref(ms))
+        return ms.value
+
+    def problem(n, seed):
+        g = torch.Generator().manual_seed(seed)
+        X = torch.rand(n, 3, generator=g, dtype=f64).to(dev)
+        z = torch.randn(n, generator=g, dtype=f64).to(dev)
+        with torch.no_grad():
+            y = ag.cholesky(ag.se_kernel(X, 0.2), NB if n == N else 256) @ z
+        return X, y
+
+    def run(X, y, nb, dtype=f64):
+        ell = torch.tensor(0.2, dtype=f64, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        K = ag.se_kernel(X, ell)
+        K.retain_grad()
+        Kd = K if dtype == f64 else K.to(dtype)
+        if dtype != f64:
+            Kd.retain_grad()
+        nll = ag.gp_nll(Kd, y.to(dtype), nb)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        potrf_ms = last_ms()
+        nll.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return nll, ell, (Kd.grad if dtype != f64 else K.grad), K.detach(), t1 - t0, t2 - t1, potrf_ms, last_ms()
+
+    def analytic(K, y):
+        Lt = torch.linalg.cholesky(K)
+        Ki = torch.cholesky_inverse(Lt)
+        al = Ki @ y
+        return 0.5 * (Ki - torch.outer(al, al))
+
+    X, y = problem(N, 5)
+    run(X, y, NB)                                  # warm-up (workspaces, code objects)
+    nll, ell, G, K, tf, tb, pm, am = run(X, y, NB)
+    Ga = analytic(K, y)
+    rel = ((G - Ga).abs().max() / Ga.abs().max()).item()
+    h = 1e-6
+    with torch.no_grad():
+        fd = (ag.gp_nll(ag.se_kernel(X, 0.2 + h), y, NB) - ag.gp_nll(ag.se_kernel(X, 0.2 - h), y, NB)).item() / (2 * h)
+    out = {"N": N, "NB": NB, "fp64": {
+        "nll": nll.item(), "forward_ms": round(tf * 1e3, 3), "backward_ms": round(tb * 1e3, 3),
+        "dpotrf_ms": round(pm, 3), "adjoint_ms": round(am, 3),
+        "adjoint_tflops": round(flops_adjoint(N, NB) / am / 1e9, 2),
+        "grad_K_rel_err_vs_analytic": rel,
+        "dnll_dlengthscale": ell.grad.item(), "fd_dnll_dlengthscale": fd,
+        "lengthscale_rel_err_vs_fd": abs(ell.grad.item() - fd) / abs(fd)}}
+    del G, Ga
+    run(X, y, NB, torch.float32)
+    nll32, ell32, G32, _, tf, tb, pm, am = run(X, y, NB, torch.float32)
+    Ga = analytic(K, y)
+    out["fp32"] = {"nll": nll32.item(), "forward_ms": round(tf * 1e3, 3), "backward_ms": round(tb * 1e3, 3),
+                   "spotrf_ms": round(pm, 3), "adjoint_ms": round(am, 3),
+                   "grad_K_rel_err_vs_fp64_analytic": ((G32.double() - Ga).abs().max() / Ga.abs().max()).item(),
+                   "lengthscale_rel_err_vs_fp64_fd": abs(ell32.grad.item() - fd) / abs(fd)}
+    del G32, Ga, K
+    print(f"[gp] N={N}: fp64 fwd {out['fp64']['forward_ms']} ms bwd {out['fp64']['backward_ms']} ms "
+          f"(adjoint {out['fp64']['adjoint_ms']} ms), grad rel err {rel:.2e}", file=sys.stderr, flush=True)
+    # a batch of GPs of nb_small points, one after another
+    worst = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in range(batch):
+        Xb, yb = problem(nb_small, 1000 + b)
+        _, _, Gb, Kb, _, _, _, _ = run(Xb, yb, 256)
+        if b < 4:
+            worst = max(worst, ((Gb - analytic(Kb, yb)).abs().max() / analytic(Kb, yb).abs().max()).item())
+    torch.cuda.synchronize()
+    out["batch"] = {"gps": batch, "N": nb_small, "NB": 256, "total_s": round(time.perf_counter() - t0, 3),
+                    "grad_K_rel_err_vs_analytic_first4": worst}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +267,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision dposv leg")
     ap.add_argument("--mixed-only", action="store_true", help="run only the mixed-precision dposv leg")
     ap.add_argument("--kappas", default="1e2,1e5,1e8", help="condition numbers of the mixed leg")
+    ap.add_argument("--no-gp", action="store_true", help="skip the Gaussian-process adjoint leg")
+    ap.add_argument("--gp-only", action="store_true", help="run only the Gaussian-process adjoint leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="one untimed factorization, for rocprofv3 runs")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -180,6 +291,9 @@ def main():
         print(json.dumps({"copy_calibration_bytes_per_launch": {"read": 1 << 30, "write": 1 << 30}}))
         return
     N, NB = args.n, args.nb
+    if args.gp_only:
+        print(json.dumps({"gp_nll": gp_leg(dp)}))
+        return
     if args.mixed_only:
         kap = tuple(float(k) for k in args.kappas.split(","))
         print(json.dumps({"mixed_dposv": mixed_leg(dp, N, NB, kappas=kap)}))
@@ -245,6 +359,14 @@ def main():
     ctx.sync_host(X)
     chk_s = ctx.check_daxmb(0, dp.LOWER, A0, B, X)
 
+    # adjoint of the same factorization: Lbar = a dplgsy-pattern matrix (seed 11), in HBM
+    Gm = A0                                      # reuse: A0 is no longer needed
+    assert ctx.dplgsy(1.0, dp.LOWER, Gm, 11) == 0
+    t0 = time.perf_counter()
+    assert ctx.dpotrf_adjoint(dp.LOWER, A, Gm) == 0
+    t_adj = time.perf_counter() - t0
+    adj_ms = ctx.last_timing()["total_ms"]
+
     line = {
         "metric": "dposv s to FP64 backward err @N=32k: mixed vs FP64 GPU; dpotrf TFLOP/s",
         "value": round(value, 3),
@@ -266,6 +388,9 @@ def main():
         "dpotrs_ms": round(t_potrs * 1e3, 3),
 
         "kernels_ms": {k: round(v[1], 3) for k, v in prof.items()},
+        "adjoint": {"ms": round(adj_ms, 3), "host_ms": round(t_adj * 1e3, 3),
+                    "flops": flops_adjoint(N, NB), "tflops": round(flops_adjoint(N, NB) / adj_ms / 1e9, 3),
+                    "vs_dpotrf_time": round(adj_ms / (total / args.steps * 1e3), 3)},
         "roofline": {
             "bound": "mfma",
             "kernel": UPDATE_KERNEL,
@@ -282,6 +407,8 @@ def main():
     ctx.close()
     if not args.no_mixed:
         line["mixed_dposv"] = mixed_leg(dp, N, NB, kappas=tuple(float(k) for k in args.kappas.split(",")))
+    if not args.no_gp:
+        line["gp_nll"] = gp_leg(dp)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))
