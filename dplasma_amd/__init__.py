@@ -127,6 +127,21 @@ def libs():
     _sig(M, "mi_strsm_LL", i, [vp, i, i, ctypes.c_float, i, i, vp, i, vp, i])
     _sig(M, "mi_spotrs_tiles", i, [vp, i, i, vp, i, vp, i])
     _sig(M, "mi_dpotrf_adjoint_tiles", i, [vp, i, i, vp, vp])
+    _sig(M, "mi_batch_sizes", i, [u64, i, i, i, pi])
+    _sig(M, "mi_batch_create", i, [vp, i, pi, ctypes.POINTER(vp)])
+    _sig(M, "mi_dbatch_spd", i, [vp, i, pi, u64, vp])
+    _sig(M, "mi_batch_destroy", i, [vp])
+    _sig(M, "mi_batch_extent", i, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)])
+    _sig(M, "mi_batch_set_shape", i, [vp, i, i])
+    _sig(M, "mi_batch_get_shape", i, [vp, pi, pi])
+    _sig(M, "mi_dbatch_autotune", i, [vp, vp, vp, vp, vp, pi])
+    _sig(M, "mi_sbatch_autotune", i, [vp, vp, vp, vp, vp, pi])
+    _sig(M, "mi_dposv_batched", i, [vp, vp, vp, i])
+    _sig(M, "mi_sposv_batched", i, [vp, vp, vp, i])
+    _sig(M, "mi_batch_info", i, [vp, pi])
+    _sig(M, "mi_dposv_batched_loop", i, [vp, vp, vp])
+    _sig(M, "mi_sposv_batched_loop", i, [vp, vp, vp])
+    _sig(M, "mi_batch_last_ms", i, [vp, pd])
     _sig(M, "mi_spotrf_adjoint_tiles", i, [vp, i, i, vp, vp])
     _sig(M, "mi_dpotrf_step_potrf", i, [vp, i, i, vp, i, pi])
     _sig(M, "mi_dpotrf_step_trsm", i, [vp, i, i, vp, i])

@@ -53,6 +53,20 @@ template <> struct MF<float> {
     __device__ static inline int row(int q, int fk) { return 4 * fk + q; }
 };
 
+// Wave-uniform copy of a value held by one lane (v_readlane_b32; two for a double).
+__device__ inline double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double readlane_t(double v, int lane) { return readlane_d(v, lane); }
+__device__ inline float readlane_t(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 struct ProfRec {
     const char *name;
     hipEvent_t ev0, ev1;

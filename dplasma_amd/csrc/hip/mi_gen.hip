@@ -18,7 +18,9 @@
 
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "mi_common.hpp"
 #include "dplasma_mi.h"
@@ -35,15 +37,17 @@ __device__ __host__ inline uint64_t splitmix64(uint64_t x)
     return z ^ (z >> 31);
 }
 
-__global__ __launch_bounds__(256) void k_gauss(double *__restrict__ G, int64_t n, uint64_t seed)
+// G[t] = element t0 + t of the Gaussian stream of seed
+__global__ __launch_bounds__(256) void k_gauss(double *__restrict__ G, int64_t n, uint64_t seed, int64_t t0)
 {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = t0 + i;
         // word w of the sequence started at seed is splitmix64(seed + w * golden)
         const uint64_t a = splitmix64(seed + (uint64_t)(2 * t) * 0x9E3779B97F4A7C15ULL);
         const uint64_t b = splitmix64(seed + (uint64_t)(2 * t + 1) * 0x9E3779B97F4A7C15ULL);
         const double u1 = (double)((a >> 11) + 1) * 0x1p-53;
         const double u2 = (double)(b >> 11) * 0x1p-53;
-        G[t] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+        G[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
     }
 }
 
@@ -78,6 +82,16 @@ __global__ __launch_bounds__(256) void k_lower_to_tiles(const double *__restrict
         if (r < N && c < N) v = (r >= c) ? C[c * ldc + r] : C[r * ldc + c];
         T[t] = v;
     }
+}
+
+// A_i(j, j) += 1 for every matrix of a batch (column-major, lda = n_i)
+__global__ __launch_bounds__(256) void k_batch_add_identity(double *__restrict__ A, const int *__restrict__ n,
+                                                           const int64_t *__restrict__ off, int count)
+{
+    const int i = blockIdx.x;
+    if (i >= count) return;
+    const int ni = n[i];
+    for (int j = threadIdx.x; j < ni; j += blockDim.x) A[off[i] + (int64_t)j * (ni + 1)] += 1.0;
 }
 
 inline unsigned grid_n(int64_t work)
@@ -136,7 +150,7 @@ extern "C" {
 int mi_dgauss(mi_ctx_t *c, int64_t n, double *dG, uint64_t seed)
 {
     if (n <= 0) return 0;
-    MI_LAUNCH(c, c->s_main, "gauss", 0.0, 8.0 * n, k_gauss<<<grid_n(n), 256, 0, c->s_main>>>(dG, n, seed));
+    MI_LAUNCH(c, c->s_main, "gauss", 0.0, 8.0 * n, k_gauss<<<grid_n(n), 256, 0, c->s_main>>>(dG, n, seed, 0));
     return mi_synchronize(c);
 }
 
@@ -149,7 +163,7 @@ int mi_dorth_haar(mi_ctx_t *c, int N, uint64_t seed, double *dQ)
     rocblas_handle h = nullptr;
     double *tau = nullptr;
     MI_CHECK(hipMalloc(&tau, (size_t)N * sizeof(double)));
-    k_gauss<<<grid_n((int64_t)N * N), 256, 0, c->s_main>>>(dQ, (int64_t)N * N, seed);
+    k_gauss<<<grid_n((int64_t)N * N), 256, 0, c->s_main>>>(dQ, (int64_t)N * N, seed, 0);
     RB_CHECK(g_roc.create(&h));
     RB_CHECK(g_roc.set_stream(h, c->s_main));
     RB_CHECK(g_roc.dgeqrf(h, N, N, dQ, N, tau));
@@ -187,6 +201,56 @@ out:
     if (rc == 0) rc = mi_synchronize(c);
     (void)hipFree(W);
     (void)hipFree(C);
+    return rc;
+}
+
+// Batch of SPD matrices A_i = G_i^T G_i / n_i + I (lower triangle written),
+// G_i n_i x n_i Gaussian: the elements of the Gaussian stream of seed taken
+// in order, G_0 first (column-major each).  Layout as mi_batch_create.
+int mi_dbatch_spd(mi_ctx_t *c, int count, const int *n, uint64_t seed, double *dA)
+{
+    if (count <= 0) return 0;
+    if (!roc_load()) return -5;
+    int rc = 0, nmax = 0;
+    rocblas_handle h = nullptr;
+    double *G = nullptr;
+    int *dn = nullptr;
+    int64_t *doff = nullptr;
+    std::vector<int64_t> off(count);
+    int64_t tot = 0;
+    for (int i = 0; i < count; i++) {
+        off[i] = tot;
+        tot += (int64_t)n[i] * n[i];
+        nmax = std::max(nmax, n[i]);
+    }
+    MI_CHECK(hipMalloc(&G, (size_t)nmax * nmax * sizeof(double)));
+    if (hipMalloc(&dn, count * sizeof(int)) != hipSuccess || hipMalloc(&doff, count * sizeof(int64_t)) != hipSuccess) {
+        set_error("out of device memory for the batch generator");
+        rc = -3;
+        goto out;
+    }
+    if (hipMemcpy(dn, n, count * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(doff, off.data(), count * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("batch generator: copy failed");
+        rc = -4;
+        goto out;
+    }
+    RB_CHECK(g_roc.create(&h));
+    RB_CHECK(g_roc.set_stream(h, c->s_main));
+    for (int i = 0; i < count; i++) {
+        const int64_t ne = (int64_t)n[i] * n[i];
+        const double alpha = 1.0 / n[i], zero = 0.0;
+        k_gauss<<<grid_n(ne), 256, 0, c->s_main>>>(G, ne, seed, off[i]);
+        RB_CHECK(g_roc.dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, n[i], n[i], &alpha, G, n[i], &zero,
+                             dA + off[i], n[i]));
+    }
+    k_batch_add_identity<<<count, 256, 0, c->s_main>>>(dA, dn, doff, count);
+out:
+    if (h) (void)g_roc.destroy(h);
+    if (rc == 0) rc = mi_synchronize(c);
+    (void)hipFree(G);
+    (void)hipFree(dn);
+    (void)hipFree(doff);
     return rc;
 }
 

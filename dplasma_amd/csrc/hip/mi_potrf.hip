@@ -87,20 +87,6 @@ __device__ inline void mfma_panel_2(const T *__restrict__ X, int ldx, int r0, in
     acc[1] += e1 + o1;
 }
 
-// ---------------------------------------------------------------------------
-// Wave-uniform copy of a double held by one lane (two v_readlane_b32).
-__device__ inline double readlane_d(double v, int lane)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ inline double readlane_t(double v, int lane) { return readlane_d(v, lane); }
-__device__ inline float readlane_t(float v, int lane)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
 
 // C(r0:r0+32, c0:c0+32) -= X(r0:, 0:32) X(c0:, 0:32)^T on the lower triangle
 // of A (rows, columns < n), X held in LDS as P[k * RC + row]; one wave, 2 x 2

@@ -177,6 +177,47 @@ int mi_dlag2s_tiles(mi_ctx_t *ctx, int64_t n, const double *dS, float *dD, int *
 int mi_dpotrf_adjoint_tiles(mi_ctx_t *ctx, int N, int NB, const double *dL, double *dLbar);
 int mi_spotrf_adjoint_tiles(mi_ctx_t *ctx, int N, int NB, const float *dL, float *dLbar);
 
+/* ------------------------------------------------------------ batched */
+/* Many independent SPD systems of uneven orders n_i in [1, 1024], each
+ * factored and solved (NRHS = 1) as LAPACK dpotrf + dpotrs would (no
+ * counterpart in the reference: its batching growth path, each matrix being
+ * the one-tile case of dplasma_dposv, src/zposv_wrapper.c:63-113 with
+ * src/cores/core_zpotrf.c:68-74).  Matrix i is column-major, lda = n_i, at
+ * element offset sum_{j<i} n_j^2 of dA; its right-hand side at offset
+ * sum_{j<i} n_j of dB (dB may be NULL: factor only).  One persistent launch
+ * works through the batch; with use_graph the launch is captured in a
+ * hipGraph on the first call for given (dA, dB) and replayed after. */
+typedef struct mi_batch_s mi_batch_t;
+/* n[i] = lo + (splitmix64 word i of seed) mod (hi - lo + 1) */
+int mi_batch_sizes(uint64_t seed, int count, int lo, int hi, int *n);
+/* Test batch: A_i = G_i^T G_i / n_i + I (lower triangles), G_i Gaussian
+ * (consecutive elements of the mi_dgauss stream of seed, G_0 first);
+ * rocBLAS dsyrk loaded at run time.  Blocking. */
+int mi_dbatch_spd(mi_ctx_t *ctx, int count, const int *n, uint64_t seed, double *dA);
+int mi_batch_create(mi_ctx_t *ctx, int count, const int *n, mi_batch_t **plan);
+int mi_batch_destroy(mi_batch_t *plan);
+int mi_batch_extent(const mi_batch_t *plan, int64_t *a_elems, int64_t *b_elems);
+/* tile shape: matrices with n <= nsplit use 64-row strips, the others
+ * 128-row strips; wg_per_cu = 1 or 2 persistent workgroups per CU */
+int mi_batch_set_shape(mi_batch_t *plan, int nsplit, int wg_per_cu);
+int mi_batch_get_shape(const mi_batch_t *plan, int *nsplit, int *wg_per_cu);
+/* Times each candidate shape on a copy of (dA0, dB0) in (dA, dB) and keeps
+ * the fastest; *choice = its index.  dA/dB are overwritten. */
+int mi_dbatch_autotune(mi_batch_t *plan, const double *dA0, const double *dB0, double *dA, double *dB,
+                       int *choice);
+int mi_sbatch_autotune(mi_batch_t *plan, const float *dA0, const float *dB0, float *dA, float *dB, int *choice);
+/* Factor (and solve when dB != NULL) in place; blocking. */
+int mi_dposv_batched(mi_batch_t *plan, double *dA, double *dB, int use_graph);
+int mi_sposv_batched(mi_batch_t *plan, float *dA, float *dB, int use_graph);
+/* The same work as one launch per matrix (the launch-loop baseline). */
+int mi_dposv_batched_loop(mi_batch_t *plan, double *dA, double *dB);
+int mi_sposv_batched_loop(mi_batch_t *plan, float *dA, float *dB);
+/* info[i] of the last run: 0, or the 1-based column of matrix i whose pivot
+ * was not positive (then no solve for it). */
+int mi_batch_info(const mi_batch_t *plan, int *info);
+/* kernel time of the last run (HIP events around the launch), ms */
+int mi_batch_last_ms(const mi_batch_t *plan, double *ms);
+
 /* ------------------------------------------------------------ checks */
 /* Norms and residuals behind check_dpotrf / check_daxmb
  * (src/dplasma_zcheck.c:63-133,179-224) computed on the GPU:
