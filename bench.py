@@ -13,7 +13,9 @@ the mixed-precision dposv at N = 32768, NRHS = 16 on A = Q diag(lambda) Q^T
 for kappa in {1e2, 1e5, 1e8} against the FP64 GPU dpotrf + dpotrs of the same
 system (time, refinement steps, backward error, error vs the FP64 answer),
 the adjoint of the factorization (df/dA from a given df/dL) at the same
-size, the differentiated Gaussian-process likelihood (N = 8192, FP64 and
+size, the batched uneven Cholesky + solve (4096 matrices of orders 32..1024,
+FP64 and FP32, one hipGraph-replayed persistent launch against a launch per
+matrix), the differentiated Gaussian-process likelihood (N = 8192, FP64 and
 FP32, plus 64 GPs of 1024 points), the roofline of the dominant kernel (the
 trailing update, timed with HIP events on its own stream in a separate
 profiled step) and the CPU baseline:
@@ -256,6 +258,101 @@ def gp_leg(dp, N=8192, NB=512, batch=64, nb_small=1024):
     return out
 
 
+def batched_leg(dp, count=4096, lo=32, hi=1024, seed=11):
+    """Batched uneven Cholesky + solve: `count` SPD matrices with orders
+    uniform in [lo, hi] (splitmix64, seed 11), A_i = G_i^T G_i / n_i + I with
+    G_i Gaussian, b uniform in [-0.5, 0.5], NRHS = 1.  FP64 and FP32 (the FP64
+    batch rounded), each as one persistent launch replayed from a hipGraph
+    with the tile shape the autotuner picked, against the same kernel
+    launched once per matrix (FP64).  Flops: LAWN-41 dpotrf + dpotrs counts
+    summed over the batch.  Backward errors of 8 matrices checked on the host."""
+    import ctypes
+    import numpy as np
+    mi, _ = dp.libs()
+    ns = (ctypes.c_int * count)()
+    mi.mi_batch_sizes(seed, count, lo, hi, ns)
+    nl = np.array(list(ns), dtype=np.float64)
+    flops = float(np.sum(nl ** 3 / 3 + nl ** 2 / 2 + nl / 6 + 2 * nl ** 2))
+    out = {"count": count, "n_range": [lo, hi], "flops": flops}
+    with dp.Device(0) as dev:
+        p = ctypes.c_void_p()
+        dev.check(mi.mi_batch_create(dev.h, count, ns, ctypes.byref(p)), "batch_create")
+        ae, be = ctypes.c_int64(), ctypes.c_int64()
+        mi.mi_batch_extent(p, ctypes.byref(ae), ctypes.byref(be))
+        A0, A, B0, B = dev.alloc(ae.value * 8), dev.alloc(ae.value * 8), dev.alloc(be.value * 8), dev.alloc(be.value * 8)
+        dev.check(mi.mi_dbatch_spd(dev.h, count, ns, seed, A0), "batch_spd")
+        dev.check(mi.mi_dplrnt(dev.h, be.value, 1, B0, be.value, be.value, 0, 0, seed), "b")
+        ms = ctypes.c_double()
+        sh_n, sh_w = ctypes.c_int(), ctypes.c_int()
+
+        def timed(fn, a0, a, b0, b, nbytes_a, nbytes_b, reps=3):
+            best = []
+            for _ in range(reps):
+                dev.check(mi.mi_memcpy_d2d(dev.h, a, a0, nbytes_a), "copy")
+                dev.check(mi.mi_memcpy_d2d(dev.h, b, b0, nbytes_b), "copy")
+                dev.check(fn(), "batched")
+                mi.mi_batch_last_ms(p, ctypes.byref(ms))
+                best.append(ms.value)
+            return sorted(best)[len(best) // 2]
+
+        def check(a0, x, b0, to64):
+            ao = np.cumsum([0] + [n * n for n in ns])
+            bo = np.cumsum([0] + list(ns))
+            worst = 0.0
+            for i in list(range(0, count, max(1, count // 8)))[:8]:
+                n = ns[i]
+                Ai = to64(a0, ao[i], n * n).reshape(n, n, order="F")
+                Ai = np.tril(Ai) + np.tril(Ai, -1).T
+                xi, bi = to64(x, bo[i], n), to64(b0, bo[i], n)
+                r = np.abs(bi - Ai @ xi).max() / (np.abs(Ai).sum(1).max() * np.abs(xi).max() + np.abs(bi).max())
+                worst = max(worst, r)
+            return worst
+
+        def dl64(ptr, off, k):
+            buf = np.empty(k)
+            dev.check(mi.mi_memcpy_d2h(dev.h, buf.ctypes.data, int(ptr) + int(off) * 8, int(k) * 8), "d2h")
+            return buf
+
+        ch = ctypes.c_int()
+        dev.check(mi.mi_dbatch_autotune(p, A0, B0, A, B, ctypes.byref(ch)), "autotune")
+        mi.mi_batch_get_shape(p, ctypes.byref(sh_n), ctypes.byref(sh_w))
+        t64 = timed(lambda: mi.mi_dposv_batched(p, A, B, 1), A0, A, B0, B, ae.value * 8, be.value * 8)
+        info = (ctypes.c_int * count)()
+        mi.mi_batch_info(p, info)
+        out["fp64"] = {"ms": round(t64, 3), "matrices_per_s": round(count / t64 * 1e3, 1),
+                       "tflops": round(flops / t64 / 1e9, 3), "shape": {"nsplit": sh_n.value, "wg_per_cu": sh_w.value},
+                       "failed": sum(1 for v in info if v),
+                       "max_backward_error_sample": check(A0, B, B0, dl64)}
+        tl = timed(lambda: mi.mi_dposv_batched_loop(p, A, B), A0, A, B0, B, ae.value * 8, be.value * 8, reps=1)
+        out["fp64_launch_loop"] = {"ms": round(tl, 3), "matrices_per_s": round(count / tl * 1e3, 1),
+                                   "tflops": round(flops / tl / 1e9, 4)}
+        out["fp64"]["speedup_vs_launch_loop"] = round(tl / t64, 2)
+        print(f"[batched] fp64 {t64:.2f} ms ({count / t64 * 1e3:.0f} matrices/s), launch loop {tl:.1f} ms",
+              file=sys.stderr, flush=True)
+        # FP32: the FP64 batch rounded
+        A32, A32w, B32, B32w = (dev.alloc(ae.value * 4), dev.alloc(ae.value * 4), dev.alloc(be.value * 4),
+                                dev.alloc(be.value * 4))
+        ovf = ctypes.c_int()
+        dev.check(mi.mi_dlag2s_tiles(dev.h, ae.value, A0, A32, ctypes.byref(ovf)), "d2s")
+        dev.check(mi.mi_dlag2s_tiles(dev.h, be.value, B0, B32, ctypes.byref(ovf)), "d2s")
+        dev.check(mi.mi_sbatch_autotune(p, A32, B32, A32w, B32w, ctypes.byref(ch)), "autotune32")
+        mi.mi_batch_get_shape(p, ctypes.byref(sh_n), ctypes.byref(sh_w))
+        t32 = timed(lambda: mi.mi_sposv_batched(p, A32w, B32w, 1), A32, A32w, B32, B32w, ae.value * 4, be.value * 4)
+
+        def dl32(ptr, off, k):
+            buf = np.empty(k, dtype=np.float32)
+            dev.check(mi.mi_memcpy_d2h(dev.h, buf.ctypes.data, int(ptr) + int(off) * 4, int(k) * 4), "d2h")
+            return buf.astype(np.float64)
+
+        out["fp32"] = {"ms": round(t32, 3), "matrices_per_s": round(count / t32 * 1e3, 1),
+                       "tflops": round(flops / t32 / 1e9, 3), "shape": {"nsplit": sh_n.value, "wg_per_cu": sh_w.value},
+                       "max_backward_error_sample": check(A32, B32w, B32, dl32)}
+        for q in (A0, A, B0, B, A32, A32w, B32, B32w):
+            dev.free(q)
+        mi.mi_batch_destroy(p)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +366,8 @@ def main():
     ap.add_argument("--kappas", default="1e2,1e5,1e8", help="condition numbers of the mixed leg")
     ap.add_argument("--no-gp", action="store_true", help="skip the Gaussian-process adjoint leg")
     ap.add_argument("--gp-only", action="store_true", help="run only the Gaussian-process adjoint leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven Cholesky leg")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched uneven Cholesky leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="one untimed factorization, for rocprofv3 runs")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -293,6 +392,9 @@ def main():
     N, NB = args.n, args.nb
     if args.gp_only:
         print(json.dumps({"gp_nll": gp_leg(dp)}))
+        return
+    if args.batched_only:
+        print(json.dumps({"batched_posv": batched_leg(dp)}))
         return
     if args.mixed_only:
         kap = tuple(float(k) for k in args.kappas.split(","))
@@ -407,6 +509,8 @@ def main():
     ctx.close()
     if not args.no_mixed:
         line["mixed_dposv"] = mixed_leg(dp, N, NB, kappas=tuple(float(k) for k in args.kappas.split(",")))
+    if not args.no_batched:
+        line["batched_posv"] = batched_leg(dp)
     if not args.no_gp:
         line["gp_nll"] = gp_leg(dp)
     if not args.no_cpu:
