@@ -149,6 +149,7 @@ def libs():
     _sig(M, "mi_set_update_engine", i, [i])
     _sig(M, "mi_set_panel_engine", i, [i])
     _sig(M, "mi_set_spotrf_update", i, [i])
+    _sig(M, "mi_set_solve_engine", i, [i])
     _sig(M, "mi_dcheck_potrf", i, [vp, i, i, vp, vp, pd, pd])
     _sig(M, "mi_dcheck_axmb", i, [vp, i, i, vp, i, vp, i, vp, i, pd, pd, pd, pd])
     _sig(M, "mi_prof_enable", i, [vp, i])

@@ -445,6 +445,232 @@ __global__ __launch_bounds__(256) void k_gemv_bwd(const T *__restrict__ Tt, int 
     }
 }
 
+// ---------------------------------------------------------------------------
+// One-launch wavefront solve (up to 16 right-hand sides, NB % 32 == 0).
+// Workgroup g (a logical index from an atomic ticket, so every lower index
+// is already running) owns rows [128 g, 128 g + 128) of X, 32 rows per wave.
+// Forward it subtracts L(rows, block h) X(block h) for every block h < g as
+// its flag appears; the L operands of the next block are loaded while the
+// workgroup waits, the X operands all at once after the flag.  Then
+// x_g = W_g y_g with W_g the inverse of the 128 x 128 diagonal triangle
+// (k_inv128, staged in LDS at the start), X is written, every wave fences
+// its stores at device scope and the flag is released.  Backward mirrors it
+// from the last block with L^T and W_g^T.  Each L element is read once per
+// sweep; per workgroup the chain costs one block update and one 128-row
+// product.  Every workgroup waits only on lower logical indices, so the
+// launch always drains.
+constexpr int WV_RB = 128;
+constexpr int WV_WS = 136;   // LDS column stride of W_g (128 + 8: fragment reads in two passes)
+
+template <class T>
+__device__ inline const T *sym_elem(const T *Tt, int MT, int NB, int row, int col)
+{
+    const int m = row / NB, n = col / NB;
+    return Tt + tile_index(MT, m, n) * (int64_t)NB * NB + (int64_t)(col - n * NB) * NB + (row - m * NB);
+}
+
+// Wait until flags[h] is set: relaxed polls (no cache maintenance per
+// iteration), then one device-scope acquire fence before the data is read.
+__device__ inline void wave_flag_wait(int *flags, int h)
+{
+    if (threadIdx.x == 0) {
+        int *fp = flags + h;
+        asm volatile("" : "+v"(fp));        // vector-memory access
+        while (__hip_atomic_load(fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// Inverse of each 128 x 128 diagonal triangle of L (column-major, 128 x 128
+// per block g, at W128 + g * 16384) from the 32 x 32 inverses W32 of
+// k_inv_diag_blocks by doubling: W21 = -W22 L21 W11, for 32 then 64 rows.
+// Rows beyond N act as identity.  One workgroup per block.
+template <class T>
+__global__ __launch_bounds__(256) void k_inv128(const T *__restrict__ Tt, int N, int NB, int MT,
+                                               const T *__restrict__ W32, T *__restrict__ W128)
+{
+    __shared__ T tmp[64 * 65];
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int r0 = g * WV_RB, nblk = NB / 32;
+    T *Wg = W128 + (int64_t)g * 16384;
+    auto L = [&](int i, int j) -> T {       // L(r0 + i, r0 + j), i > j, 0 beyond N
+        return (r0 + i < N) ? *sym_elem(Tt, MT, NB, r0 + i, r0 + j) : T(0);
+    };
+    for (int e = tid; e < 16384; e += 256) {
+        const int i = e & 127, j = e >> 7;
+        const int bi = i >> 5, bj = j >> 5;
+        T v = T(0);
+        if (bi == bj) {
+            const int rc = r0 + 32 * bi;
+            if (rc < N) {
+                const int tk = rc / NB, cb = (rc - tk * NB) / 32;
+                v = W32[((int64_t)tk * nblk + cb) * 1024 + (j & 31) * 32 + (i & 31)];
+            } else {
+                v = (i == j) ? T(1) : T(0);
+            }
+        }
+        Wg[j * 128 + i] = v;
+    }
+    __syncthreads();
+    for (int h = 32; h < 128; h *= 2) {
+        for (int p = 0; p < 128; p += 2 * h) {
+            const int q = p + h;
+            // tmp = L(q.., p..) W(p.., p..)   (h x h)
+            for (int e = tid; e < h * h; e += 256) {
+                const int i = e % h, j = e / h;
+                T a = T(0);
+                for (int k = j; k < h; k++) a += L(q + i, p + k) * Wg[(p + j) * 128 + p + k];
+                tmp[j * 65 + i] = a;
+            }
+            __syncthreads();
+            // W(q.., p..) = -W(q.., q..) tmp
+            for (int e = tid; e < h * h; e += 256) {
+                const int i = e % h, j = e / h;
+                T a = T(0);
+                for (int k = 0; k <= i; k++) a += Wg[(q + k) * 128 + q + i] * tmp[j * 65 + k];
+                Wg[(p + j) * 128 + q + i] = -a;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <class T, bool TRANS>
+__global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, int N, int NB, int MT,
+                                                     const T *__restrict__ W128, T *__restrict__ B, int ldb,
+                                                     int nr, int *__restrict__ flags, int *__restrict__ ticket)
+{
+    typedef typename MF<T>::v4 V4;
+    __shared__ T Ws[128 * WV_WS];    // a-operands of the diagonal product: Ws[k * WV_WS + i] = op(W)(i, k)
+    __shared__ T Ys[128][17];
+    __shared__ int s_g;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int fr = lane & 15, fk = lane >> 4;
+    const int nwg = (N + WV_RB - 1) / WV_RB;
+    if (tid == 0) {
+        int *tp = ticket;
+        asm volatile("" : "+v"(tp));
+        s_g = atomicAdd(tp, 1);
+    }
+    __syncthreads();
+    const int g = TRANS ? nwg - 1 - s_g : s_g;
+    const int r0 = g * WV_RB;
+    const int rw = r0 + 32 * wave;               // this wave's 32 rows
+    // Loads below always use in-bounds addresses (indices clamped into the
+    // tile or matrix) and mask the value afterwards, so they issue together
+    // instead of one branch and one wait each.
+    {
+        const T *Wg = W128 + (int64_t)g * 16384;
+        for (int e0 = 0; e0 < 16384; e0 += 256 * 16) {
+            T v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int e = e0 + 256 * u + tid, i = e & 127, k = e >> 7;
+                v[u] = TRANS ? Wg[i * 128 + k] : Wg[k * 128 + i];
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int e = e0 + 256 * u + tid, i = e & 127, k = e >> 7;
+                Ws[k * WV_WS + i] = v[u];
+            }
+        }
+    }
+    const int frc = min(fr, max(nr, 1) - 1);
+    V4 acc[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int row = rw + 16 * t + MF<T>::row(q, fk);
+            const T v = B[(int64_t)frc * ldb + min(row, N - 1)];
+            acc[t][q] = (row < N && fr < nr) ? v : T(0);
+        }
+
+    // a-operands of one other block h: op(L)(my 32 rows, the block's 128 columns)
+    T la[4][8][2];
+    auto load_a = [&](int h) {
+#pragma unroll
+        for (int hb = 0; hb < 4; hb++) {
+            const int c0 = h * WV_RB + 32 * hb;
+            const int cc = min(c0, N - 1) & ~31;     // a valid 32-aligned block when c0 >= N
+            const int rr = min(rw, N - 1) & ~31;
+            const T *pA = TRANS ? sym_elem(Tt, MT, NB, cc, rr) : sym_elem(Tt, MT, NB, rr, cc);
+#pragma unroll
+            for (int kq = 0; kq < 8; kq++) {
+                const int kk = 4 * kq + fk;
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int i = 16 * t + fr;
+                    const T v = TRANS ? pA[(int64_t)i * NB + kk] : pA[(int64_t)kk * NB + i];
+                    la[hb][kq][t] = (c0 < N && rw + i < N && c0 + kk < N) ? v : T(0);
+                }
+            }
+        }
+    };
+    const int nprev = TRANS ? nwg - 1 - g : g;
+    if (nprev > 0 && rw < N) load_a(TRANS ? nwg - 1 : 0);
+    for (int s = 0; s < nprev; s++) {
+        const int h = TRANS ? nwg - 1 - s : s;
+        wave_flag_wait(flags, h);
+        if (rw < N) {
+            T bx[4][8];                           // -X(block h): the update subtracts
+#pragma unroll
+            for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+                for (int kq = 0; kq < 8; kq++) {
+                    const int r = h * WV_RB + 32 * hb + 4 * kq + fk;
+                    const T v = B[(int64_t)frc * ldb + min(r, N - 1)];
+                    bx[hb][kq] = (fr < nr && r < N) ? -v : T(0);
+                }
+#pragma unroll
+            for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+                for (int kq = 0; kq < 8; kq++)
+#pragma unroll
+                    for (int t = 0; t < 2; t++) acc[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], acc[t]);
+            if (s + 1 < nprev) load_a(TRANS ? nwg - 2 - s : s + 1);
+        }
+    }
+
+    // x_g = op(W_g) y_g
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) Ys[32 * wave + 16 * t + MF<T>::row(q, fk)][fr] = acc[t][q];
+    __syncthreads();
+    V4 x[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) x[t] = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll 8
+    for (int kq = 0; kq < 32; kq++) {
+        const int kk = 4 * kq + fk;
+        const T b = Ys[kk][fr];
+#pragma unroll
+        for (int t = 0; t < 2; t++) x[t] = MF<T>::mfma(Ws[kk * WV_WS + 32 * wave + 16 * t + fr], b, x[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int row = rw + 16 * t + MF<T>::row(q, fk);
+            if (row < N && fr < nr) B[(int64_t)fr * ldb + row] = x[t][q];
+        }
+    // every wave's X stores complete, then one device-scope release of the flag
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) {
+        int *fp = flags + g;
+        asm volatile("" : "+v"(fp));
+        __hip_atomic_store(fp, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_wave_reset(int *__restrict__ flags, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) flags[i] = 0;
+}
+
 template <class T>
 __global__ void k_scale(T *__restrict__ B, int N, int nrhs, int ldb, T alpha)
 {
@@ -472,6 +698,20 @@ int ensure_sws(size_t bytes)
     return 0;
 }
 
+SolveWs g_wflags;
+int g_wave_solve = 1;   // 1: one-launch wavefront solve when NB % 32 == 0; 0: per-tile launches
+
+int ensure_wflags(size_t bytes)
+{
+    if (g_wflags.bytes >= bytes) return 0;
+    if (g_wflags.w) (void)hipFree(g_wflags.w);
+    g_wflags.w = nullptr;
+    g_wflags.bytes = 0;
+    MI_CHECK(hipMalloc(&g_wflags.w, bytes));
+    g_wflags.bytes = bytes;
+    return 0;
+}
+
 int set_lds(const void *f, size_t bytes)
 {
     if (bytes > 65536) MI_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -481,11 +721,11 @@ int set_lds(const void *f, size_t bytes)
 template <class T> struct SN;
 template <> struct SN<double> {
     static constexpr const char *diag = "dtrsm_diag", *gemv = "dtrsm_gemv", *inv = "dtrsm_inv_diag",
-                                *scale = "dtrsm_scale";
+                                *scale = "dtrsm_scale", *wave = "dtrsm_wave";
 };
 template <> struct SN<float> {
     static constexpr const char *diag = "strsm_diag", *gemv = "strsm_gemv", *inv = "strsm_inv_diag",
-                                *scale = "strsm_scale";
+                                *scale = "strsm_scale", *wave = "strsm_wave";
 };
 
 template <class T, int NR>
@@ -559,7 +799,9 @@ namespace mi {
 size_t trsm_ws_elems(int N, int NB)
 {
     const int MT = (N + NB - 1) / NB;
-    return (size_t)MT * ((NB + 31) / 32) * 1024;
+    const size_t w32 = (size_t)MT * ((NB + 31) / 32) * 1024;
+    // plus the 128 x 128 inverses of the wavefront solve
+    return w32 + (NB % 32 == 0 ? (size_t)((N + WV_RB - 1) / WV_RB) * 16384 : 0);
 }
 
 // Inverses of the 32x32 diagonal blocks of every diagonal tile into W
@@ -571,6 +813,12 @@ int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int NB, const T *dT, T 
     const int nblk = (NB + 31) / 32;
     MI_LAUNCH(c, s, SN<T>::inv, (double)MT * NB * 32 * 32 / 3.0, (double)MT * NB * 32 * 2.0 * sizeof(T),
               k_inv_diag_blocks<T><<<MT * nblk, 32, 0, s>>>(dT, N, NB, MT, unit, W));
+    if (NB % 32 == 0 && g_wave_solve) {
+        const int nwg = (N + WV_RB - 1) / WV_RB;
+        T *W128 = W + (size_t)MT * nblk * 1024;
+        MI_LAUNCH(c, s, SN<T>::inv, (double)nwg * 128 * 128 * 128 / 3.0, (double)nwg * 16384 * 2.0 * sizeof(T),
+                  k_inv128<T><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W, W128));
+    }
     return 0;
 }
 
@@ -579,8 +827,26 @@ template <class T>
 int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, const T *W, int NRHS, T *dB,
                  int ldb)
 {
-    if (NB > TSV_MAX_NB) { set_error("tile size NB=%d above the solve limit %d", NB, TSV_MAX_NB); return -4; }
     const int MT = (N + NB - 1) / NB;
+    if (NB % 32 == 0 && g_wave_solve) {
+        // one wavefront launch per chunk of 16 right-hand sides
+        const int nwg = (N + WV_RB - 1) / WV_RB;
+        int rc;
+        if ((rc = ensure_wflags((size_t)(nwg + 1) * sizeof(int)))) return rc;
+        int *flags = (int *)g_wflags.w;
+        const T *W128 = W + (size_t)MT * ((NB + 31) / 32) * 1024;
+        for (int j0 = 0; j0 < NRHS; j0 += RHSB) {
+            const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
+            T *Bj = dB + (int64_t)j0 * ldb;
+            k_wave_reset<<<1, 256, 0, s>>>(flags, nwg);
+            MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
+                      (trans == 111
+                       ? k_trsm_wave<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Bj, ldb, nr, flags, flags + nwg)
+                       : k_trsm_wave<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Bj, ldb, nr, flags, flags + nwg)));
+        }
+        return 0;
+    }
+    if (NB > TSV_MAX_NB) { set_error("tile size NB=%d above the solve limit %d", NB, TSV_MAX_NB); return -4; }
     for (int j0 = 0; j0 < NRHS; j0 += RHSB) {
         const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
         T *Bj = dB + (int64_t)j0 * ldb;
@@ -661,4 +927,13 @@ extern "C" int mi_dpotrs_tiles(mi_ctx_t *c, int N, int NB, const double *dT, int
 extern "C" int mi_spotrs_tiles(mi_ctx_t *c, int N, int NB, const float *dT, int NRHS, float *dB, int ldb)
 {
     return potrs_tiles<float>(c, N, NB, dT, NRHS, dB, ldb);
+}
+
+// 1 (default): the one-launch wavefront solve whenever NB % 32 == 0;
+// 0: one diagonal-solve and one update launch per tile.  Process-wide.
+extern "C" int mi_set_solve_engine(int which)
+{
+    if (which != 0 && which != 1) { set_error("solve engine must be 0 or 1"); return -2; }
+    g_wave_solve = which;
+    return 0;
 }
