@@ -1224,10 +1224,18 @@ int ensure_ws(size_t bytes)
     return 0;
 }
 
-// Whole factorization of the packed lower tiles dT (N, NB), resident.
-//   for k: SYRK(k, k+1) -> POTRF(k+1) and the rest of column k+1 -> TRSM(k+1)
-//   on the panel stream, the rest of the trailing update of step k on the
-//   main stream (src/zpotrf_L.jdf, one-panel look-ahead).
+// Whole factorization of the packed lower tiles dT (N, NB), resident, with a
+// two-step look-ahead over two streams (src/zpotrf_L.jdf's priorities give
+// the panel the critical path):
+//   panel stream, step k:  [wait: column k+1 updated by step k-1]
+//                          column k+1 updated by step k (SYRK + GEMMs),
+//                          POTRF(k+1), TRSM(k+1)
+//   main stream, step k:   [wait: TRSM(k)]
+//                          column k+2 updated by step k  -> event for the panel,
+//                          columns k+3.. updated by step k (the bulk)
+// Every tile gets its updates in step order and no two launches touch the
+// same tile at once; the panel chain waits for one column of the bulk of
+// step k-1, not for all of it.
 template <class T>
 int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info)
 {
@@ -1252,28 +1260,29 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info)
     // panel 0
     if ((rc = launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, 0, ws))) return rc;
     if ((rc = launch_trsm<T>(c, c->s_panel, dT, N, NB, MT, 0, ws))) return rc;
-    hipEvent_t pk = next_event(c);
+    hipEvent_t pk = next_event(c);                 // TRSM(k) done
     MI_CHECK(hipEventRecord(pk, c->s_panel));
+    hipEvent_t col_ready = nullptr;                // column k+1 updated by step k-1
 
     for (int k = 0; k < MT - 1; k++) {
-        MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
-        // the next diagonal tile first: POTRF(k+1) only waits for its SYRK
-        if ((rc = launch_update_run<T>(c, c->s_main, dT, N, NB, MT, k, k + 1, 0, 1))) return rc;
-        hipEvent_t dk = next_event(c);
-        MI_CHECK(hipEventRecord(dk, c->s_main));
-        // then the rest of column k+1, which gates TRSM(k+1)
-        if ((rc = launch_update_run<T>(c, c->s_main, dT, N, NB, MT, k, k + 1, 1, MT - k - 2))) return rc;
-        hipEvent_t ck = next_event(c);
-        MI_CHECK(hipEventRecord(ck, c->s_main));
-        MI_CHECK(hipStreamWaitEvent(c->s_panel, dk, 0));
+        // panel stream: column k+1 by step k, then the next panel
+        if (col_ready) MI_CHECK(hipStreamWaitEvent(c->s_panel, col_ready, 0));
+        if ((rc = launch_update<T>(c, c->s_panel, dT, N, NB, MT, k, k + 1, k + 2))) return rc;
         T *w = ws + ((k + 1) & 1) * wsz;
         if ((rc = launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
-        MI_CHECK(hipStreamWaitEvent(c->s_panel, ck, 0));
         if ((rc = launch_trsm<T>(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
-        pk = next_event(c);
-        MI_CHECK(hipEventRecord(pk, c->s_panel));
-        // the rest of the trailing matrix overlaps the panel above
-        if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 2, MT))) return rc;
+        hipEvent_t pk_next = next_event(c);
+        MI_CHECK(hipEventRecord(pk_next, c->s_panel));
+        // main stream: the look-ahead column k+2, then the bulk of step k
+        MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
+        col_ready = nullptr;
+        if (k + 2 < MT) {
+            if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 2, k + 3))) return rc;
+            col_ready = next_event(c);
+            MI_CHECK(hipEventRecord(col_ready, c->s_main));
+        }
+        if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 3, MT))) return rc;
+        pk = pk_next;
     }
     MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
     MI_CHECK(hipEventRecord(c->t1, c->s_main));
