@@ -454,7 +454,7 @@ __global__ __launch_bounds__(256) void k_gemv_bwd(const T *__restrict__ Tt, int 
 // workgroup waits, the X operands all at once after the flag.  Then
 // x_g = W_g y_g with W_g the inverse of the 128 x 128 diagonal triangle
 // (k_inv128, staged in LDS at the start), X is written, every wave fences
-// its stores at device scope and the flag is released.  Backward mirrors it
+// its stores and the flag is set.  Backward mirrors it
 // from the last block with L^T and W_g^T.  Each L element is read once per
 // sweep; per workgroup the chain costs one block update and one 128-row
 // product.  Every workgroup waits only on lower logical indices, so the
@@ -469,15 +469,21 @@ __device__ inline const T *sym_elem(const T *Tt, int MT, int NB, int row, int co
     return Tt + tile_index(MT, m, n) * (int64_t)NB * NB + (int64_t)(col - n * NB) * NB + (row - m * NB);
 }
 
-// Wait until flags[h] is set: relaxed polls (no cache maintenance per
-// iteration), then one device-scope acquire fence before the data is read.
+// Hand-off between workgroups (message passing through uncached memory):
+// the flags and the staging copy xs of X live in uncached device memory, so
+// every load and store reaches memory; the producer waits for its xs stores
+// to complete before it stores its flag, and the consumer issues its xs
+// loads only after its flag load returned the set value.  (Release/acquire
+// fences on cached memory would write back and invalidate the whole L2 at
+// every hand-off instead.)  xs holds block g's 128 rows x 16 columns at
+// xs + g * 2048, row-major by column index: xs[g * 2048 + j * 128 + i].
+
 __device__ inline void wave_flag_wait(int *flags, int h)
 {
     if (threadIdx.x == 0) {
         int *fp = flags + h;
         asm volatile("" : "+v"(fp));        // vector-memory access
         while (__hip_atomic_load(fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) __builtin_amdgcn_s_sleep(1);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
@@ -539,7 +545,8 @@ __global__ __launch_bounds__(256) void k_inv128(const T *__restrict__ Tt, int N,
 template <class T, bool TRANS>
 __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, int N, int NB, int MT,
                                                      const T *__restrict__ W128, T *__restrict__ B, int ldb,
-                                                     int nr, int *__restrict__ flags, int *__restrict__ ticket)
+                                                     int nr, T *__restrict__ xs, int *__restrict__ flags,
+                                                     int *__restrict__ ticket)
 {
     typedef typename MF<T>::v4 V4;
     __shared__ T Ws[128 * WV_WS];    // a-operands of the diagonal product: Ws[k * WV_WS + i] = op(W)(i, k)
@@ -620,7 +627,7 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
 #pragma unroll
                 for (int kq = 0; kq < 8; kq++) {
                     const int r = h * WV_RB + 32 * hb + 4 * kq + fk;
-                    const T v = B[(int64_t)frc * ldb + min(r, N - 1)];
+                    const T v = xs[(int64_t)h * (WV_RB * RHSB) + fr * WV_RB + 32 * hb + 4 * kq + fk];
                     bx[hb][kq] = (fr < nr && r < N) ? -v : T(0);
                 }
 #pragma unroll
@@ -654,15 +661,16 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int row = rw + 16 * t + MF<T>::row(q, fk);
+            xs[(int64_t)g * (WV_RB * RHSB) + fr * WV_RB + 32 * wave + 16 * t + MF<T>::row(q, fk)] = x[t][q];
             if (row < N && fr < nr) B[(int64_t)fr * ldb + row] = x[t][q];
         }
-    // every wave's X stores complete, then one device-scope release of the flag
+    // every wave's X stores complete, then the flag
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
         int *fp = flags + g;
         asm volatile("" : "+v"(fp));
-        __hip_atomic_store(fp, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(fp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -698,17 +706,21 @@ int ensure_sws(size_t bytes)
     return 0;
 }
 
-SolveWs g_wflags;
+// Hand-off memory of the wavefront solve: the flags (and ticket) and a
+// staging copy of X, in uncached device memory so that every access reaches
+// memory and a value stored by one XCD is visible to the others without cache
+// maintenance.  Uncached allocations are never freed once a kernel has read
+// them (a grown buffer replaces the old one, which stays allocated).
+SolveWs g_wflags, g_wstage;
 int g_wave_solve = 1;   // 1: one-launch wavefront solve when NB % 32 == 0; 0: per-tile launches
 
-int ensure_wflags(size_t bytes)
+int ensure_uncached(SolveWs &w, size_t bytes)
 {
-    if (g_wflags.bytes >= bytes) return 0;
-    if (g_wflags.w) (void)hipFree(g_wflags.w);
-    g_wflags.w = nullptr;
-    g_wflags.bytes = 0;
-    MI_CHECK(hipMalloc(&g_wflags.w, bytes));
-    g_wflags.bytes = bytes;
+    if (w.bytes >= bytes) return 0;
+    void *p = nullptr;
+    MI_CHECK(hipExtMallocWithFlags(&p, bytes, hipDeviceMallocUncached));
+    w.w = p;
+    w.bytes = bytes;
     return 0;
 }
 
@@ -832,8 +844,10 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
         // one wavefront launch per chunk of 16 right-hand sides
         const int nwg = (N + WV_RB - 1) / WV_RB;
         int rc;
-        if ((rc = ensure_wflags((size_t)(nwg + 1) * sizeof(int)))) return rc;
+        if ((rc = ensure_uncached(g_wflags, (size_t)(nwg + 1) * sizeof(int)))) return rc;
+        if ((rc = ensure_uncached(g_wstage, (size_t)nwg * WV_RB * RHSB * sizeof(T)))) return rc;
         int *flags = (int *)g_wflags.w;
+        T *xs = (T *)g_wstage.w;
         const T *W128 = W + (size_t)MT * ((NB + 31) / 32) * 1024;
         for (int j0 = 0; j0 < NRHS; j0 += RHSB) {
             const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
@@ -841,8 +855,10 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
             k_wave_reset<<<1, 256, 0, s>>>(flags, nwg);
             MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
                       (trans == 111
-                       ? k_trsm_wave<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Bj, ldb, nr, flags, flags + nwg)
-                       : k_trsm_wave<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Bj, ldb, nr, flags, flags + nwg)));
+                       ? k_trsm_wave<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Bj, ldb, nr, xs, flags,
+                                                                  flags + nwg)
+                       : k_trsm_wave<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Bj, ldb, nr, xs, flags,
+                                                                 flags + nwg)));
         }
         return 0;
     }
