@@ -370,6 +370,9 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="run only the batched uneven Cholesky leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="one untimed factorization, for rocprofv3 runs")
+    ap.add_argument("--profile-kind", default="dpotrf", choices=["dpotrf", "spotrf", "spotrf_bf16x3", "dpotrs"],
+                    help="what --profile-only runs: the FP64 factorization, the FP32 factorization (FP32 "
+                         "MFMA or BF16-split updates) of the same matrix rounded, or the FP64 solve (16 RHS)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the plain copy kernel on 1 GiB (scales HBM counters)")
     args = ap.parse_args()
@@ -399,6 +402,30 @@ def main():
     if args.mixed_only:
         kap = tuple(float(k) for k in args.kappas.split(","))
         print(json.dumps({"mixed_dposv": mixed_leg(dp, N, NB, kappas=kap)}))
+        return
+    if args.profile_only and args.profile_kind != "dpotrf":
+        import ctypes
+        mi, _ = dp.libs()
+        with dp.Device(0) as dev:
+            MT = (N + NB - 1) // NB
+            nt = MT * (MT + 1) // 2 * NB * NB
+            dA = dev.alloc(nt * 8)
+            dev.check(mi.mi_dplgsy_tiles(dev.h, float(N), N, NB, dA, 3872), "dplgsy")
+            info = ctypes.c_int()
+            if args.profile_kind == "dpotrs":
+                dev.check(mi.mi_dpotrf_tiles(dev.h, N, NB, dA, ctypes.byref(info)), "dpotrf")
+                dB = dev.alloc(N * 16 * 8)
+                dev.check(mi.mi_dplrnt(dev.h, N, 16, dB, N, N, 0, 0, 3873), "B")
+                dev.check(mi.mi_dpotrs_tiles(dev.h, N, NB, dA, 16, dB, N), "dpotrs")
+            else:
+                d32 = dev.alloc(nt * 4)
+                ovf = ctypes.c_int()
+                dev.check(mi.mi_dlag2s_tiles(dev.h, nt, dA, d32, ctypes.byref(ovf)), "dlag2s")
+                mi.mi_set_spotrf_update(1 if args.profile_kind == "spotrf_bf16x3" else 0)
+                dev.check(mi.mi_spotrf_tiles(dev.h, N, NB, d32, ctypes.byref(info)), "spotrf")
+                mi.mi_set_spotrf_update(0)
+            assert info.value == 0
+        print("profile step done")
         return
     if args.profile_only:
         ctx = dp.Context(0)
