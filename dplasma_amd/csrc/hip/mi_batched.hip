@@ -17,8 +17,8 @@
 // factors one matrix with all its waves, left-looking over 32-column panels:
 //   1. panel update C -= L(j:n, 0:j) L(j:j+32, 0:j)^T on MFMA, strips of SH
 //      rows staged through LDS in K chunks of 32;
-//   2. the 32x32 diagonal block factored by wave 0 in registers (one
-//      v_readlane broadcast per column) and inverted (W = L_jj^-1);
+//   2. the 32x32 diagonal block factored and inverted (W = L_jj^-1) by wave
+//      0 in one register sweep (one v_readlane broadcast per column);
 //   3. the rows below: X = C W^T on MFMA;
 // then the two triangular solves, 32 rows at a time (partial sums by all
 // waves, the 32-row substitution by wave 0).
@@ -49,7 +49,6 @@ template <class T>
 struct BShared {
     T sa[BW * (128 + 128 / sizeof(T))];   // -L rows of the strip, [kk][row]
     T sb[BW * (BW + 128 / sizeof(T))];    // L rows of the diagonal block, [kk][col]
-    T ls[BW][BW + 1];                     // factored diagonal block
     T ws[BW][BW + 1];                     // its inverse
     T ys[BMAX];                           // solve vector
     T red[8][BW];
@@ -139,8 +138,11 @@ __device__ void panel_update(T *__restrict__ A, int n, int j0, int jb, BShared<T
     }
 }
 
-// 2. factor the diagonal block (wave 0, registers) and invert it; returns
-// the 1-based column of the first non-positive pivot within the block, or 0
+// 2. factor the diagonal block and invert it in one register sweep of wave
+// 0: lanes 0-31 hold rows of the block and become L_jj, lanes 32-63 hold
+// the rows of the identity and become the columns of W = L_jj^-1 (the same
+// eliminations applied to [L | I]).  Returns the 1-based column of the first
+// non-positive pivot within the block, or 0.
 template <class T>
 __device__ __forceinline__ int diag_block(T *__restrict__ A, int n, int j0, int jb, BShared<T> &sh)
 {
@@ -152,11 +154,13 @@ __device__ __forceinline__ int diag_block(T *__restrict__ A, int n, int j0, int 
         // opaque to the optimizer: keeps the 32 identity tests below from
         // being hoisted out of the caller's panel loop into live registers
         asm volatile("" : "+v"(r));
+        const bool lrow = r < BW;
+        const int jw = r - BW;
         T v[BW];
 #pragma unroll
         for (int c = 0; c < BW; c++) {
-            T x = (r == c) ? T(1) : T(0);
-            if (r < jb && c < jb && c <= r) x = A[(j0 + c) * lda + j0 + r];
+            T x = lrow ? ((r == c) ? T(1) : T(0)) : ((c == jw) ? T(1) : T(0));
+            if (lrow && r < jb && c < jb && c <= r) x = A[(j0 + c) * lda + j0 + r];
             v[c] = x;
         }
 #pragma unroll
@@ -164,36 +168,26 @@ __device__ __forceinline__ int diag_block(T *__restrict__ A, int n, int j0, int 
             const T piv = readlane_t(v[c], c);
             if (!(piv > T(0)) && fail == 0) fail = c + 1;
             const T d = sqrt(piv > T(0) ? piv : T(1));
-            if (r == c) v[c] = d;
-            else if (r > c) v[c] = v[c] / d;
-            const T m = (r > c) ? v[c] : T(0);
+            const T rd = T(1) / d;
+            const T sc = v[c] * rd;
+            const T m = (!lrow || r > c) ? sc : T(0);
+            v[c] = lrow ? (r == c ? d : (r > c ? sc : v[c])) : sc;
 #pragma unroll
             for (int c2 = c + 1; c2 < BW; c2++) v[c2] -= m * readlane_t(m, c2);
         }
         if (r == 0) sh.fail = (fail > 0 && fail <= jb) ? fail : 0;   // wave-uniform
-        if (r < BW) {
-#pragma unroll
-            for (int c = 0; c < BW; c++) sh.ls[r][c] = (c <= r) ? v[c] : T(0);
+        if (lrow) {
             if (r < jb) {
 #pragma unroll
                 for (int c = 0; c < BW; c++)
                     if (c <= r && c < jb) A[(j0 + c) * lda + j0 + r] = v[c];
             }
+        } else {
+#pragma unroll
+            for (int c = 0; c < BW; c++) sh.ws[c][jw] = v[c];      // W(c, jw)
         }
     }
     __syncthreads();
-    if (tid < BW) {                               // W = L_jj^-1, lane c forms column c
-        const int c = tid;
-        for (int i = 0; i < BW; i++) {
-            T w = T(0);
-            if (i >= c) {
-                T sacc = (i == c) ? T(1) : T(0);
-                for (int p = c; p < i; p++) sacc -= sh.ls[i][p] * sh.ws[p][c];
-                w = sacc / sh.ls[i][i];
-            }
-            sh.ws[i][c] = w;
-        }
-    }
     const int f = sh.fail;
     __syncthreads();
     return f;
@@ -277,8 +271,18 @@ __device__ void solve_one(const T *__restrict__ A, int n, T *__restrict__ b, BSh
         {
             const int r = tid & 31, sl = tid >> 5;
             T acc = T(0);
-            if (r < jb)
-                for (int k = sl; k < j0; k += 8) acc += A[k * lda + j0 + r] * sh.ys[k];
+            if (r < jb) {
+                // eight independent loads in flight per step
+                int k = sl;
+                for (; k + 56 < j0; k += 64) {
+                    T a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) a[u] = A[(k + 8 * u) * lda + j0 + r];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) acc += a[u] * sh.ys[k + 8 * u];
+                }
+                for (; k < j0; k += 8) acc += A[k * lda + j0 + r] * sh.ys[k];
+            }
             sh.red[sl][r] = acc;
         }
         __syncthreads();
@@ -290,10 +294,18 @@ __device__ void solve_one(const T *__restrict__ A, int n, T *__restrict__ b, BSh
                 for (int sl = 0; sl < 8; sl++) t -= sh.red[sl][r];
                 dg = A[(j0 + r) * lda + j0 + r];
             }
-            for (int c = 0; c < jb; c++) {
-                const T yc = readlane_t(t / dg, c);
-                if (r == c) t = yc;
-                else if (r > c && r < jb) t -= A[(j0 + c) * lda + j0 + r] * yc;
+            // the block's row r, loaded before the sequential substitution
+            T Lr[BW];
+#pragma unroll
+            for (int c = 0; c < BW; c++) Lr[c] = (r < jb && c < r) ? A[(j0 + c) * lda + j0 + r] : T(0);
+            const T idg = T(1) / dg;
+#pragma unroll
+            for (int c = 0; c < BW; c++) {
+                if (c < jb) {
+                    const T yc = readlane_t(t * idg, c);
+                    if (r == c) t = yc;
+                    else if (r > c) t -= Lr[c] * yc;
+                }
             }
             if (r < jb) sh.ys[j0 + r] = t;
         }
@@ -305,8 +317,17 @@ __device__ void solve_one(const T *__restrict__ A, int n, T *__restrict__ b, BSh
         {
             const int r = tid >> 3, sl = tid & 7;
             T acc = T(0);
-            if (r < jb)
-                for (int k = j0 + jb + sl; k < n; k += 8) acc += A[(j0 + r) * lda + k] * sh.ys[k];
+            if (r < jb) {
+                int k = j0 + jb + sl;
+                for (; k + 56 < n; k += 64) {
+                    T a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) a[u] = A[(j0 + r) * lda + k + 8 * u];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) acc += a[u] * sh.ys[k + 8 * u];
+                }
+                for (; k < n; k += 8) acc += A[(j0 + r) * lda + k] * sh.ys[k];
+            }
             sh.red[sl][r] = acc;
         }
         __syncthreads();
@@ -318,10 +339,18 @@ __device__ void solve_one(const T *__restrict__ A, int n, T *__restrict__ b, BSh
                 for (int sl = 0; sl < 8; sl++) t -= sh.red[sl][r];
                 dg = A[(j0 + r) * lda + j0 + r];
             }
-            for (int c = jb - 1; c >= 0; c--) {
-                const T xc = readlane_t(t / dg, c);
-                if (r == c) t = xc;
-                else if (r < c) t -= A[(j0 + r) * lda + j0 + c] * xc;
+            // the block's column r (L(c, r), c > r), loaded before the substitution
+            T Lc[BW];
+#pragma unroll
+            for (int c = 0; c < BW; c++) Lc[c] = (r < jb && c > r && c < jb) ? A[(j0 + r) * lda + j0 + c] : T(0);
+            const T idg = T(1) / dg;
+#pragma unroll
+            for (int c = BW - 1; c >= 0; c--) {
+                if (c < jb) {
+                    const T xc = readlane_t(t * idg, c);
+                    if (r == c) t = xc;
+                    else if (r < c) t -= Lc[c] * xc;
+                }
             }
             if (r < jb) sh.ys[j0 + r] = t;
         }
