@@ -127,6 +127,8 @@ def libs():
     _sig(M, "mi_strsm_LL", i, [vp, i, i, ctypes.c_float, i, i, vp, i, vp, i])
     _sig(M, "mi_spotrs_tiles", i, [vp, i, i, vp, i, vp, i])
     _sig(M, "mi_dpotrf_adjoint_tiles", i, [vp, i, i, vp, vp])
+    _sig(M, "mi_dpotri_tiles", i, [vp, i, i, vp, pi])
+    _sig(M, "mi_dcheck_poinv", i, [vp, i, i, vp, vp, pd, pd, pd])
     _sig(M, "mi_batch_sizes", i, [u64, i, i, i, pi])
     _sig(M, "mi_batch_create", i, [vp, i, pi, ctypes.POINTER(vp)])
     _sig(M, "mi_dbatch_spd", i, [vp, i, pi, u64, vp])
@@ -183,6 +185,9 @@ def libs():
     _sig(D, "dplasma_dsposv", i, [vp, i, vp, vp, vp, pi])
     _sig(D, "dplasma_dposv_mixed", i, [vp, i, vp, vp, vp, i, pi])
     _sig(D, "dplasma_dpotrf_adjoint", i, [vp, i, vp, vp])
+    _sig(D, "dplasma_dpotri", i, [vp, i, vp])
+    _sig(D, "dplasma_dpoinv", i, [vp, i, vp])
+    _sig(D, "check_dpoinv", i, [vp, i, i, vp, vp])
     _sig(D, "dplasma_dtrsm", i, [vp, i, i, i, i, d, vp, vp])
     _sig(D, "dplasma_dtrsm_New", vp, [i, i, i, i, d, vp, vp])
     _sig(D, "dplasma_dtrsm_Destruct", None, [vp])
@@ -348,6 +353,15 @@ class Context:
         it = ctypes.c_int(0)
         info = self.D.dplasma_dposv_mixed(self.ptr, uplo, A.ptr, B.ptr, X.ptr, prec, ctypes.byref(it))
         return info, it.value
+
+    def dpotri(self, uplo, A):
+        return self.D.dplasma_dpotri(self.ptr, uplo, A.ptr)
+
+    def dpoinv(self, uplo, A):
+        return self.D.dplasma_dpoinv(self.ptr, uplo, A.ptr)
+
+    def check_dpoinv(self, loud, uplo, A, Ainv):
+        return self.D.check_dpoinv(self.ptr, loud, uplo, A.ptr, Ainv.ptr)
 
     def dpotrf_adjoint(self, uplo, L, Lbar):
         """Lbar (df/dL) -> symmetric df/dA in place; returns 0 or an error code."""

@@ -666,9 +666,84 @@ int adjoint_tiles(mi_ctx_t *c, int N, int NB, const T *dL, T *dG)
     return 0;
 }
 
+// Lbar = diag(2 / L_ii) (the gradient of logdet A = 2 sum log L_ii) in
+// packed lower tiles, zero elsewhere; *info = 1-based index of the first
+// zero diagonal entry of L (atomicMin over a device word holding N + 1).
+__global__ __launch_bounds__(256) void k_logdet_seed(const double *__restrict__ L, int N, int NB, int MT,
+                                                    double *__restrict__ G, int *__restrict__ zero_at)
+{
+    // one pass, one value per element (no ordering between workgroups needed)
+    const int64_t tot = (int64_t)MT * (MT + 1) / 2 * NB * NB;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t = e / ((int64_t)NB * NB);
+        const int64_t w = e - t * NB * NB;
+        const int c = (int)(w / NB), r = (int)(w - (int64_t)c * NB);
+        int n = 0;
+        while (t >= MT - n) { t -= MT - n; n++; }
+        const bool diag = (t == 0) && (r == c) && (n * NB + r < N);
+        double v = 0.0;
+        if (diag) {
+            const double d = L[e];
+            if (d == 0.0) atomicMin(zero_at, n * NB + r + 1);
+            v = 2.0 / d;
+        }
+        G[e] = v;
+    }
+}
+
+// D = S on the lower part of the packed lower tiles (diagonal tiles: lower
+// triangle only; padding rows untouched)
+__global__ __launch_bounds__(256) void k_copy_lower_tiles(const double *__restrict__ S, double *__restrict__ D, int N,
+                                                         int NB, int MT)
+{
+    const int64_t tot = (int64_t)MT * (MT + 1) / 2 * NB * NB;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t = e / ((int64_t)NB * NB);
+        const int64_t w = e - t * NB * NB;
+        const int c = (int)(w / NB), r = (int)(w - (int64_t)c * NB);
+        int n = 0;
+        while (t >= MT - n) { t -= MT - n; n++; }
+        const int m = n + (int)t;
+        if (m * NB + r >= N || n * NB + c >= N) continue;
+        if (m == n && r < c) continue;
+        D[e] = S[e];
+    }
+}
+
+DevBuf g_pinv;
+
 }  // namespace
 
 extern "C" {
+
+// In-place inverse of the SPD matrix from its Cholesky factor: the adjoint
+// of logdet A = 2 sum log L_ii, whose gradient with respect to A is A^-1.
+int mi_dpotri_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
+{
+    *info = 0;
+    if (N <= 0) return 0;
+    if (NB <= 0) { set_error("bad tile size"); return -2; }
+    const int MT = (N + NB - 1) / NB;
+    const int64_t tot = (int64_t)MT * (MT + 1) / 2 * NB * NB;
+    double *G = (double *)g_pinv.get((size_t)tot * sizeof(double));
+    if (!G) { set_error("out of device memory for the inverse"); return -3; }
+    int h_zero = 0;
+    k_fill<int><<<1, 64, 0, c->s_main>>>(c->d_info, 1, N + 1);
+    k_logdet_seed<<<(unsigned)std::min<int64_t>(65536, (tot + 255) / 256), 256, 0, c->s_main>>>(dT, N, NB, MT, G,
+                                                                                              c->d_info);
+    MI_CHECK(hipGetLastError());
+    MI_CHECK(hipMemcpyAsync(&h_zero, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
+    int rc = mi_synchronize(c);
+    if (rc) return rc;
+    if (h_zero <= N) { *info = h_zero; return 0; }      // singular factor: LAPACK dtrtri's info
+    if ((rc = adjoint_tiles<double>(c, N, NB, dT, G))) return rc;
+    const double ms = c->last_ms;
+    k_copy_lower_tiles<<<(unsigned)std::min<int64_t>(65536, (tot + 255) / 256), 256, 0, c->s_main>>>(G, dT, N, NB, MT);
+    MI_CHECK(hipGetLastError());
+    rc = mi_synchronize(c);
+    c->last_ms = ms;
+    return rc;
+}
 
 int mi_dpotrf_adjoint_tiles(mi_ctx_t *c, int N, int NB, const double *dL, double *dLbar)
 {

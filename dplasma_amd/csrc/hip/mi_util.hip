@@ -12,6 +12,8 @@
 // computed on the device (R = L L^T - A0 on the lower tiles, R = B - A0 X by
 // the strip residual kernel of mi_mixed.hip, infinity norms of symmetric and
 // general matrices by row-sum atomics).
+#include <algorithm>
+
 #include "mi_common.hpp"
 #include "dplasma_mi.h"
 
@@ -292,6 +294,38 @@ __global__ __launch_bounds__(256) void k_dcopy(const double2 *__restrict__ s, do
         d[i] = s[i];
 }
 
+// symmetric matrix in packed lower tiles -> full N x N column-major (ld N)
+__global__ __launch_bounds__(256) void k_sym_to_full(const double *__restrict__ T, int N, int NB, int MT,
+                                                    double *__restrict__ F)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)N * N;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e / N), i = (int)(e - (int64_t)j * N);
+        const int r = i >= j ? i : j, c = i >= j ? j : i;      // lower-triangle element
+        const int m = r / NB, n = c / NB;
+        F[e] = T[tile_index(MT, m, n) * (int64_t)NB * NB + (int64_t)(c - n * NB) * NB + (r - m * NB)];
+    }
+}
+
+// column sums of |A| (M x Nc, ld lda)
+__global__ __launch_bounds__(256) void k_ge_colsum(const double *__restrict__ A, int M, int Nc, int64_t lda,
+                                                  double *__restrict__ cs)
+{
+    for (int j = blockIdx.x; j < Nc; j += gridDim.x) {
+        __shared__ double sm[256];
+        double a = 0.0;
+        for (int i = threadIdx.x; i < M; i += 256) a += fabs(A[(int64_t)j * lda + i]);
+        sm[threadIdx.x] = a;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) cs[j] = sm[0];
+        __syncthreads();
+    }
+}
+
 __global__ void k_add_diag(double *__restrict__ A, int n, int lda, double alpha)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -447,6 +481,38 @@ int mi_dcheck_axmb(mi_ctx_t *c, int N, int NB, const double *dA0, int NRHS, cons
     if ((rc = dmax_of(c, rs, N, Xnorm))) goto out;
     k_ge_rowsum<<<grid_for(N), 256, 0, s>>>(r, N, NRHS, (int)npad, rs);
     if ((rc = dmax_of(c, rs, N, Rnorm))) goto out;
+out:
+    (void)hipFree(buf);
+    if (rc) return rc;
+    return mi_synchronize(c);
+}
+
+// Norms behind check_dpoinv (src/dplasma_zcheck.c:264-314), on the GPU:
+// ||A0||_1, ||Ainv||_1 (both symmetric, packed lower tiles) and
+// ||I - Ainv A0||_1, with Ainv A0 formed by the symmetric residual kernel.
+int mi_dcheck_poinv(mi_ctx_t *c, int N, int NB, const double *dA0, const double *dAinv, double *Anorm,
+                    double *Ainvnorm, double *Rnorm)
+{
+    if (N <= 0) { *Anorm = *Ainvnorm = *Rnorm = 0.0; return 0; }
+    const int MT = (N + NB - 1) / NB;
+    const int64_t npad = (int64_t)MT * NB;
+    const int64_t nn = (int64_t)N * N;
+    double *buf = nullptr;
+    // A0 full (N x N), I (N x N), R (npad x N), one vector
+    MI_CHECK(hipMalloc(&buf, (2 * nn + npad * N + npad) * sizeof(double)));
+    double *F = buf, *I = buf + nn, *R = I + nn, *v = R + npad * N;
+    hipStream_t s = c->s_main;
+    int rc;
+    k_sym_to_full<<<grid_for(nn), 256, 0, s>>>(dA0, N, NB, MT, F);
+    k_zero<<<grid_for(nn + npad * N + npad), 256, 0, s>>>(I, nn + npad * N + npad);
+    k_add_diag<<<grid_for(N), 256, 0, s>>>(I, N, N, 1.0);
+    if ((rc = dsymm_resid(c, s, N, NB, dAinv, N, F, N, I, N, R, (int)npad, v))) goto out;
+    if ((rc = dmax_of(c, v, N, Ainvnorm))) goto out;     // row sums of |Ainv| (symmetric: one-norm)
+    k_zero<<<grid_for(npad), 256, 0, s>>>(v, npad);
+    k_sym_rowsum<<<grid_for((int64_t)MT * (MT + 1) / 2 * NB), 256, 0, s>>>(dA0, N, NB, MT, v);
+    if ((rc = dmax_of(c, v, N, Anorm))) goto out;
+    k_ge_colsum<<<(unsigned)std::min(N, 65535), 256, 0, s>>>(R, N, N, npad, v);
+    if ((rc = dmax_of(c, v, N, Rnorm))) goto out;
 out:
     (void)hipFree(buf);
     if (rc) return rc;

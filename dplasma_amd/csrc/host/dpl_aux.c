@@ -167,6 +167,29 @@ int check_dpotrf(parsec_context_t *parsec, int loud, dplasma_enum_t uplo, parsec
     return bad ? 1 : 0;
 }
 
+/* check_dpoinv (src/dplasma_zcheck.c:264-314): ||I - Ainv A||_1 /
+ * (||A||_1 ||Ainv||_1 N eps) < 10, computed on the GPU.  A and Ainv are
+ * symmetric descriptors with their uplo part stored (the reference takes a
+ * fully generated A). */
+int check_dpoinv(parsec_context_t *parsec, int loud, dplasma_enum_t uplo, parsec_tiled_matrix_t *A,
+                 parsec_tiled_matrix_t *Ainv)
+{
+    dpl_sym_dev_t sa, si;
+    int N = A->n;
+    if (dpl_sym_acquire(parsec, A, uplo, 1, &sa, NULL)) return 1;
+    if (dpl_sym_acquire(parsec, Ainv, uplo, 1, &si, NULL)) { dpl_sym_release(parsec, A, &sa, 0, NULL); return 1; }
+    double Anorm = 0.0, Ainvnorm = 0.0, Rnorm = 0.0;
+    int rc = mi_dcheck_poinv(parsec->mi, sa.N, sa.NB, sa.d, si.d, &Anorm, &Ainvnorm, &Rnorm);
+    dpl_sym_release(parsec, Ainv, &si, 0, NULL);
+    dpl_sym_release(parsec, A, &sa, 0, NULL);
+    if (rc) { dpl_mi_fail("check_dpoinv", rc); return 1; }
+    double result = Rnorm / ((Anorm * Ainvnorm) * N * eps_d());
+    if (loud > 2)
+        printf("  ||A||_one = %e, ||A^(-1)||_one = %e, ||I - A * A^(-1)||_one = %e, result = %e\n", Anorm,
+               Ainvnorm, Rnorm, result);
+    return (isinf(Ainvnorm) || isnan(result) || isinf(result) || result > 10.0) ? 1 : 0;
+}
+
 int check_daxmb(parsec_context_t *parsec, int loud, dplasma_enum_t uplo, parsec_tiled_matrix_t *A,
                 parsec_tiled_matrix_t *b, parsec_tiled_matrix_t *x)
 {

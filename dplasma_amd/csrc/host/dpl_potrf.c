@@ -331,3 +331,56 @@ int dplasma_dpotrf_adjoint(parsec_context_t *parsec, dplasma_enum_t uplo, const 
     dpl_sym_release(parsec, L, &sl, 0, NULL);
     return rc ? rc : r2;
 }
+
+/* dplasma_zpotri (src/zpotri_wrapper.c:54-82): A holds the Cholesky factor
+ * of dplasma_dpotrf (uplo part) and receives the uplo part of A^-1.  The
+ * reference runs trtri then lauum; here the inverse is the adjoint of
+ * logdet A with respect to A (L^-T L^-1 from the same tile kernels).
+ * Returns -1 for an illegal uplo, i > 0 when L(i,i) is exactly zero (dtrtri's
+ * info; A unchanged), 0 otherwise. */
+int dplasma_dpotri(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A)
+{
+    if (uplo != dplasmaUpper && uplo != dplasmaLower) {
+        dpl_error("dplasma_dpotri", "illegal value of uplo");
+        return -1;
+    }
+    dpl_sym_dev_t s;
+    memset(&parsec->last, 0, sizeof parsec->last);
+    int rc = dpl_sym_acquire(parsec, A, uplo, 1, &s, &parsec->last.stage_in_ms);
+    if (rc) return rc;
+    int info = 0;
+    rc = mi_dpotri_tiles(parsec->mi, s.N, s.NB, s.d, &info);
+    if (rc) rc = dpl_mi_fail("dplasma_dpotri", rc);
+    else mi_last_span_ms(parsec->mi, &parsec->last.total_ms);
+    int r2 = dpl_sym_release(parsec, A, &s, rc == 0 && info == 0, &parsec->last.stage_out_ms);
+    return rc ? rc : (r2 ? r2 : info);
+}
+
+/* dplasma_zpoinv (src/zpoinv_wrapper.c:109-135): factor and invert in place;
+ * returns dplasma_dpotrf's info (the inverse is formed only when it is 0),
+ * -1 for an illegal uplo. */
+int dplasma_dpoinv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A)
+{
+    if (uplo != dplasmaUpper && uplo != dplasmaLower) {
+        dpl_error("dplasma_dpoinv", "illegal value of uplo");
+        return -1;
+    }
+    dpl_sym_dev_t s;
+    memset(&parsec->last, 0, sizeof parsec->last);
+    int rc = dpl_sym_acquire(parsec, A, uplo, 1, &s, &parsec->last.stage_in_ms);
+    if (rc) return rc;
+    int info = 0;
+    double ms_f = 0.0, ms_i = 0.0;
+    rc = mi_dpotrf_tiles(parsec->mi, s.N, s.NB, s.d, &info);
+    if (!rc) mi_last_span_ms(parsec->mi, &ms_f);
+    if (!rc && info == 0) {
+        int tinfo = 0;
+        rc = mi_dpotri_tiles(parsec->mi, s.N, s.NB, s.d, &tinfo);
+        if (!rc) mi_last_span_ms(parsec->mi, &ms_i);
+        if (!rc && tinfo) info = tinfo;
+    }
+    if (rc) rc = dpl_mi_fail("dplasma_dpoinv", rc);
+    parsec->last.total_ms = ms_f + ms_i;
+    int r2 = dpl_sym_release(parsec, A, &s, rc == 0, &parsec->last.stage_out_ms);
+    return rc ? rc : (r2 ? r2 : info);
+}

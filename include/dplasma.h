@@ -56,6 +56,10 @@ int dplasma_dpotrs(parsec_context_t *parsec, dplasma_enum_t uplo,
 int dplasma_dposv(parsec_context_t *parsec, dplasma_enum_t uplo,
                   parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B);
 
+/* ---------------- inverse from the Cholesky factor (dplasma_z.h:79,85) */
+int dplasma_dpotri(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A);
+int dplasma_dpoinv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_matrix_t *A);
+
 /* ---------------- mixed precision (extension: the precision growth path of
  * dplasma_dposv, src/zposv_wrapper.c:63-113; no counterpart in the reference).
  * Solves A X = B for SPD A (FP64 descriptors) with a low-precision factor
@@ -106,6 +110,9 @@ int check_dpotrf(parsec_context_t *parsec, int loud, dplasma_enum_t uplo,
                  parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *A0);
 int check_daxmb(parsec_context_t *parsec, int loud, dplasma_enum_t uplo,
                 parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *b, parsec_tiled_matrix_t *x);
+/* src/dplasma_zcheck.c:264-314; A and Ainv symmetric descriptors (uplo part) */
+int check_dpoinv(parsec_context_t *parsec, int loud, dplasma_enum_t uplo,
+                 parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *Ainv);
 
 /* ---------------- data placement (src/dplasmaaux.h:147-151) */
 /* Makes the uplo part of A resident in HBM until parsec_devices_release_memory()

@@ -181,6 +181,11 @@ int mi_dlag2s_tiles(mi_ctx_t *ctx, int64_t n, const double *dS, float *dD, int *
  * Blocking; returns 0 or a negative error. */
 int mi_dpotrf_adjoint_tiles(mi_ctx_t *ctx, int N, int NB, const double *dL, double *dLbar);
 int mi_spotrf_adjoint_tiles(mi_ctx_t *ctx, int N, int NB, const float *dL, float *dLbar);
+/* A^-1 in place of its Cholesky factor (lower tiles), as the adjoint of
+ * logdet A: the inverse step of dplasma_zpotri (src/zpotri_wrapper.c:54-82,
+ * trtri + lauum there).  *info = i > 0 if L(i,i) == 0 (dT unchanged).
+ * Blocking. */
+int mi_dpotri_tiles(mi_ctx_t *ctx, int N, int NB, double *dT, int *info);
 
 /* ------------------------------------------------------------ batched */
 /* Many independent SPD systems of uneven orders n_i in [1, 1024], each
@@ -224,6 +229,10 @@ int mi_batch_info(const mi_batch_t *plan, int *info);
 int mi_batch_last_ms(const mi_batch_t *plan, double *ms);
 
 /* ------------------------------------------------------------ checks */
+/* check_dpoinv norms (src/dplasma_zcheck.c:264-314): ||A0||_1, ||Ainv||_1
+ * of symmetric lower tiles and ||I - Ainv A0||_1.  Blocking. */
+int mi_dcheck_poinv(mi_ctx_t *ctx, int N, int NB, const double *dA0, const double *dAinv,
+                    double *Anorm, double *Ainvnorm, double *Rnorm);
 /* Norms and residuals behind check_dpotrf / check_daxmb
  * (src/dplasma_zcheck.c:63-133,179-224) computed on the GPU:
  *   *Anorm = ||A0||_oo (symmetric, lower tiles), *Rnorm = ||L L^T - A0||_oo. */

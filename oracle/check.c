@@ -129,3 +129,41 @@ double orc_check_daxmb(int uplo, int N, int NRHS, const double *A0, int lda0,
     free(r);
     return Rnorm / ((Anorm * Xnorm + Bnorm) * N * orc_dlamch_eps());
 }
+
+/* check_zpoinv (src/dplasma_zcheck.c:264-314): ||I - Ainv A||_1 /
+ * (||A||_1 ||Ainv||_1 N eps), A and Ainv symmetric with their uplo
+ * triangle referenced (the reference forms Ainv A with zhemm(Left, uplo),
+ * then zlanhe / zlange one-norms).  For a symmetric matrix the one-norm
+ * equals the infinity norm. */
+double orc_check_dpoinv(int uplo, int N, const double *A, int lda, const double *Ainv, int ldi,
+                        double *Anorm_out, double *Ainvnorm_out, double *Rnorm_out)
+{
+    double *Af = (double *)malloc(sizeof(double) * (size_t)N * N);
+    double *If = (double *)malloc(sizeof(double) * (size_t)N * N);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            const int lo = (uplo == ORC_LOWER) ? (i >= j) : (i <= j);
+            Af[(int64_t)j * N + i] = lo ? A[(int64_t)j * lda + i] : A[(int64_t)i * lda + j];
+            If[(int64_t)j * N + i] = lo ? Ainv[(int64_t)j * ldi + i] : Ainv[(int64_t)i * ldi + j];
+        }
+    double Rnorm = 0.0;
+    double *col = (double *)malloc(sizeof(double) * (size_t)N);
+    for (int j = 0; j < N; j++) {
+        for (int i = 0; i < N; i++) col[i] = (i == j) ? 1.0 : 0.0;
+        for (int p = 0; p < N; p++) {
+            const double a = Af[(int64_t)j * N + p];
+            const double *ip = If + (int64_t)p * N;
+            for (int i = 0; i < N; i++) col[i] -= ip[i] * a;
+        }
+        double cs = 0.0;
+        for (int i = 0; i < N; i++) cs += fabs(col[i]);
+        if (cs > Rnorm || cs != cs) Rnorm = cs;
+    }
+    double Anorm = orc_dlansy_inf(uplo, N, A, lda);
+    double Ainvnorm = orc_dlansy_inf(uplo, N, Ainv, ldi);
+    free(Af); free(If); free(col);
+    if (Anorm_out) *Anorm_out = Anorm;
+    if (Ainvnorm_out) *Ainvnorm_out = Ainvnorm;
+    if (Rnorm_out) *Rnorm_out = Rnorm;
+    return Rnorm / ((Anorm * Ainvnorm) * N * orc_dlamch_eps());
+}

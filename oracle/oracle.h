@@ -119,6 +119,8 @@ double orc_dlange_inf(int M, int N, const double *A, int lda);
 double orc_check_dpotrf(int uplo, int N, const double *A, int lda,
                         const double *A0, int lda0, double *Anorm, double *Rnorm);
 /* ||b - A0 x||_oo / ((||A0||_oo ||x||_oo + ||b||_oo) N eps) */
+double orc_check_dpoinv(int uplo, int N, const double *A, int lda, const double *Ainv, int ldi,
+                        double *Anorm_out, double *Ainvnorm_out, double *Rnorm_out);
 double orc_check_daxmb(int uplo, int N, int NRHS, const double *A0, int lda0,
                        const double *b, int ldb, const double *x, int ldx);
 

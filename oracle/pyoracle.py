@@ -76,6 +76,8 @@ def lib():
         "orc_dlange_inf": (_d, [_i, _i, _dp, _i]),
         "orc_check_dpotrf": (_d, [_i, _i, _dp, _i, _dp, _i, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
         "orc_check_daxmb": (_d, [_i, _i, _i, _dp, _i, _dp, _i, _dp, _i]),
+        "orc_check_dpoinv": (_d, [_i, _i, _dp, _i, _dp, _i, ctypes.POINTER(_d), ctypes.POINTER(_d),
+                                  ctypes.POINTER(_d)]),
         "orc_wtime": (_d, []),
     }
     for name, (res, args) in sig.items():
@@ -175,6 +177,14 @@ def check_dpotrf(uplo, A, A0):
     r = lib().orc_check_dpotrf(uplo, N, np.asfortranarray(A), N, np.asfortranarray(A0), N,
                                ctypes.byref(an), ctypes.byref(rn))
     return r
+
+
+def check_dpoinv(uplo, A, Ainv):
+    """orc_check_dpoinv: ||I - Ainv A||_1 / (||A||_1 ||Ainv||_1 N eps)."""
+    N = A.shape[0]
+    an, ain, rn = _d(), _d(), _d()
+    return lib().orc_check_dpoinv(uplo, N, np.asfortranarray(A), N, np.asfortranarray(Ainv), N,
+                                  ctypes.byref(an), ctypes.byref(ain), ctypes.byref(rn))
 
 
 def check_daxmb(uplo, A0, b, x):

@@ -284,3 +284,47 @@ def test_dpotrf_adjoint_bad_args(ctx, dp):
     assert ctx.dpotrf_adjoint(dp.UPPERLOWER, A, A) == -1
     assert ctx.dpotrf_adjoint(dp.LOWER, A, G) == -2
     A.free(); G.free()
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("N,NB", [(378, 93), (1000, 192), (1700, 320)])
+def test_dpoinv(ctx, dp, po, uplo, N, NB):
+    """dplasma_dpoinv (factor + inverse) on dplgsy: against numpy's inverse of
+    the same matrix (relative 1e-12), the reference's check_dpoinv < 10 on the
+    GPU and in the oracle, and dplasma_dpotri after dplasma_dpotrf giving the
+    same inverse bit for bit."""
+    u = dp.LOWER if uplo == "L" else dp.UPPER
+    pu = po.LOWER if uplo == "L" else po.UPPER
+    A = dp.SymMatrix(ctx, N, NB, u); ctx.dplgsy(float(N), u, A, 3872)
+    A0 = dp.SymMatrix(ctx, N, NB, u); ctx.dplgsy(float(N), u, A0, 3872)
+    assert ctx.dpoinv(u, A) == 0
+    full = po.dplgsy_full(N, float(N), 3872)
+    S = po.symtiles_to_full(A.array, N, NB, pu)
+    got = np.tril(S) + np.tril(S, -1).T if uplo == "L" else np.triu(S) + np.triu(S, 1).T
+    want = np.linalg.inv(full)
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    assert ctx.check_dpoinv(0, u, A0, A) == 0
+    assert po.check_dpoinv(pu, full, got) < 10
+    B = dp.SymMatrix(ctx, N, NB, u); ctx.dplgsy(float(N), u, B, 3872)
+    assert ctx.dpotrf(u, B) == 0
+    assert ctx.dpotri(u, B) == 0
+    assert np.array_equal(B.array, A.array)
+    A.free(); A0.free(); B.free()
+
+
+def test_dpoinv_args_and_singular(ctx, dp, po):
+    A = dp.SymMatrix(ctx, 200, 64, dp.LOWER); ctx.dplgsy(200.0, dp.LOWER, A, 3872)
+    assert ctx.dpotri(dp.UPPERLOWER, A) == -1
+    assert ctx.dpoinv(123, A) == -1
+    # a factor with an exact zero at L(70,70): dtrtri's info = 71, A unchanged
+    assert ctx.dpotrf(dp.LOWER, A) == 0
+    S = po.symtiles_to_full(A.array, 200, 64, po.LOWER)
+    S[70, 70] = 0.0
+    A.array[:] = po.full_to_symtiles(S, 64, po.LOWER)
+    before = A.array.copy()
+    assert ctx.dpotri(dp.LOWER, A) == 71
+    assert np.array_equal(A.array, before)
+    # not SPD: dplasma_dpoinv returns dplasma_dpotrf's info
+    M = dp.SymMatrix(ctx, 200, 64, dp.LOWER); ctx.dplgsy(0.0, dp.LOWER, M, 3872)
+    assert ctx.dpoinv(dp.LOWER, M) > 0
+    A.free(); M.free()

@@ -247,3 +247,26 @@ def test_optimized_blas_path_agrees(po):
     assert po.dpotrf_tiles(T, N, NB, nthreads=4, use_blas=True) == 0
     assert po.dpotrf_tiles(T1, N, NB, nthreads=4, use_blas=False) == 0
     assert np.abs(T - T1).max() < 1e-12
+
+
+def test_check_dpoinv_pinned(po):
+    """orc_check_dpoinv (src/dplasma_zcheck.c:264-314): exact inverses give ~0
+    for both uplo; a hand-computed 2x2 case; a planted 1e-9 error and a
+    transposed non-symmetric 'inverse' fail the < 10 threshold."""
+    I = np.eye(7)
+    assert po.check_dpoinv(po.LOWER, I, I) == 0.0
+    A = np.array([[4.0, 2.0], [2.0, 3.0]])
+    Ai = np.array([[3.0, -2.0], [-2.0, 4.0]]) / 8.0
+    assert po.check_dpoinv(po.LOWER, A, Ai) < 1.0 and po.check_dpoinv(po.UPPER, A, Ai) < 1.0
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((40, 40))
+    A = B @ B.T / 40 + np.eye(40)
+    Ai = np.linalg.inv(A)
+    assert po.check_dpoinv(po.LOWER, A, Ai) < 10
+    bad = Ai.copy()
+    bad[5, 2] += 1e-9
+    assert po.check_dpoinv(po.LOWER, A, bad) > 10
+    # only the uplo triangle is referenced: garbage above the diagonal is ignored
+    junk = np.tril(Ai) + np.triu(np.full_like(Ai, 7.0), 1)
+    assert po.check_dpoinv(po.LOWER, A, junk) < 10
+    assert po.check_dpoinv(po.UPPER, A, junk) > 10
