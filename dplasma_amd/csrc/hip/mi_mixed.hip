@@ -16,11 +16,11 @@
 //
 // Kernels here: FP64 <-> FP32 rounding (tiles and column-major blocks),
 // the FP64 symmetric residual R = B - A X over the lower tiles, and column
-// infinity norms.  The residual is deterministic: one workgroup owns a
-// 64-row strip of R and walks tile row m (tiles (m, n), n <= m) and tile
-// column m transposed (tiles (n, m), n > m); every A element is read twice
-// per residual (once per side of the diagonal), 64-row x 64 chunks staged
-// through LDS so that both sides are read coalesced.
+// infinity norms.  The residual is deterministic.  For 64 <= NB <= 512 every
+// stored A element is read once: one workgroup per tile forms A_mn X_n and
+// A_mn^T X_m on FP64 MFMA into per-tile partials, and a second pass sums the
+// partials of each row in a fixed order.  Other tiles use a strip kernel
+// that reads every element twice (once per side of the diagonal).
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -196,6 +196,206 @@ __global__ __launch_bounds__(256) void k_dsymm_resid(const double *__restrict__ 
     }
 }
 
+// The same residual with every stored element of A read once (64 <= NB <= 512),
+// in two launches.  k_symm_part: one workgroup per stored tile (m, n),
+// m >= n, computes on FP64 MFMA (16x16x4, the 16 columns are the right-hand
+// sides)
+//     P1(m, n) = A_mn X_n        (NB x 16, rows of tile m)
+//     P2(m, n) = A_mn^T X_m      (NB x 16, rows of tile n)
+// and, when asked, the row and column sums of |A_mn|.  On the diagonal
+// tile only the lower triangle is stored: P1 takes it with the diagonal,
+// P2 the strict lower part, so P1 + P2 is the full symmetric product.  Wave
+// w owns rows [128 w, 128 w + 128) of the tile for P1.  Per 32-column chunk
+// each wave reads its 16 x 32 A blocks coalesced (MFMA a-operand layout:
+// lane 16 k + i holds A(i, k)); the a-operands of P2 (rows along k) are the
+// same values transposed through a per-wave LDS buffer.  The four waves'
+// P2 contributions to a chunk are summed in LDS in a fixed order.
+// k_symm_sum then forms R = B - sum_n P1(m, n) - sum_m' P2(m', m) for each
+// row, in a fixed order, so the result is deterministic.
+// Partials per tile: P1 [NB][16], P2 [NB][16], |A| row sums [NB], column
+// sums [NB] -- 34 NB doubles at tile_index(m, n) * 34 NB.
+constexpr int SY_TS = 17;      // column stride of the per-wave transpose buffer (doubles)
+constexpr int SY_PART = 34;    // partial doubles per tile row index
+
+template <bool ABS>
+__global__ __launch_bounds__(256, 2) void k_symm_part(const double *__restrict__ A, int N, int NB, int MT,
+                                                     const double *__restrict__ X, int ldx, int nr,
+                                                     double *__restrict__ P)
+{
+    constexpr int CW = 32;                       // columns per chunk
+    __shared__ double Tb[4][CW * SY_TS];
+    __shared__ double red[4][CW * 16 + CW];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int fr = lane & 15, fk = lane >> 4;
+    int t = blockIdx.x, n = 0;
+    while (t >= MT - n) { t -= MT - n; n++; }
+    const int m = n + t;
+    const bool dg = (m == n);
+    const int rows_m = tile_rows(N, NB, MT, m), cols_n = tile_rows(N, NB, MT, n);
+    const int64_t ti = tile_index(MT, m, n);
+    const double *At = A + ti * (int64_t)NB * NB;
+    double *P1 = P + ti * (int64_t)SY_PART * NB;
+    double *P2 = P1 + 16 * NB;
+    double *A1 = P2 + 16 * NB;
+    double *A2 = A1 + NB;
+    const int jc = min(fr, nr - 1);              // clamped right-hand side of this lane
+    const bool jok = fr < nr;
+    const double *Xn = X + (int64_t)jc * ldx + (int64_t)n * NB;
+    const double *Xm = X + (int64_t)jc * ldx + (int64_t)m * NB;
+    double *T = Tb[w];
+    const int ng = min(8, max(0, (rows_m - 128 * w + 15) / 16));   // row groups of this wave (uniform)
+
+    dx4 p1[8];
+    double a1[8];
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+        p1[g] = dx4{0.0, 0.0, 0.0, 0.0};
+        a1[g] = 0.0;
+    }
+    // operands of row group g of chunk c0: a-operands of P1 A(128 w + 16 g + fr,
+    // c0 + 4 u + fk) (32-bit offsets from the tile base) and b-operands of P2
+    // X_m(128 w + 16 g + 4 s + fk, fr).  The row is made opaque per call, so
+    // that row-dependent addresses are not precomputed for all row groups.
+    auto load_grp = [&](double (&a)[8], double (&xm)[4], int c0, int g) {
+        int rb = 128 * w + 16 * g;
+        asm volatile("" : "+v"(rb));
+        const int r = rb + fr;
+        const unsigned rc = (unsigned)min(r, rows_m - 1);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = c0 + 4 * u + fk;
+            const double v = At[(unsigned)min(c, cols_n - 1) * (unsigned)NB + rc];
+            // in the tile, and in the lower triangle on the diagonal tile
+            const int ok = (rows_m - 1 - r) | (cols_n - 1 - c) | (dg ? r - c : 0);
+            a[u] = ok >= 0 ? v : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int rr = rb + 4 * s + fk;
+            const double v = Xm[(unsigned)min(rr, rows_m - 1)];
+            xm[s] = (jok && rr < rows_m) ? v : 0.0;
+        }
+    };
+    const int nch = (NB + CW - 1) / CW;
+    double a[8], xm[4];
+    if (ng > 0) load_grp(a, xm, 0, 0);
+    for (int ch = 0; ch < nch; ch++) {
+        const int c0 = CW * ch;
+        // b-operands of P1: X_n(c0 + 4 u + fk, fr)
+        double xn[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = c0 + 4 * u + fk;
+            const double v = Xn[(unsigned)min(c, cols_n - 1)];
+            xn[u] = (jok && c < cols_n) ? v : 0.0;
+        }
+        dx4 p2[2];
+        double a2[2];
+#pragma unroll
+        for (int cg = 0; cg < 2; cg++) {
+            p2[cg] = dx4{0.0, 0.0, 0.0, 0.0};
+            a2[cg] = 0.0;
+        }
+#pragma unroll
+        for (int g = 0; g < 8; g++) {
+            if (g < ng) {                         // wave-uniform
+                // the next row group's operands (across the chunk boundary too)
+                // are in flight while this one is used
+                asm volatile("" ::: "memory");
+                double an[8], xmn[4];
+                if (g + 1 < ng) load_grp(an, xmn, c0, g + 1);
+                else if (ch + 1 < nch) load_grp(an, xmn, c0 + CW, 0);
+                const int r0 = 128 * w + 16 * g;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    p1[g] = MF<double>::mfma(a[u], xn[u], p1[g]);
+                    if (ABS) a1[g] += fabs(a[u]);
+                    T[(4 * u + fk) * SY_TS + fr] = a[u];
+                }
+#pragma unroll
+                for (int cg = 0; cg < 2; cg++)
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        // A(r0 + 4 s + fk, c0 + 16 cg + fr); strict lower part on the diagonal tile
+                        double v = T[(16 * cg + fr) * SY_TS + 4 * s + fk];
+                        if (dg && !(r0 + 4 * s + fk > c0 + 16 * cg + fr)) v = 0.0;
+                        p2[cg] = MF<double>::mfma(v, xm[s], p2[cg]);
+                        if (ABS) a2[cg] += fabs(v);
+                    }
+#pragma unroll
+                for (int u = 0; u < 8; u++) a[u] = an[u];
+#pragma unroll
+                for (int s = 0; s < 4; s++) xm[s] = xmn[s];
+            }
+        }
+        // element q of p2[cg] in lane (fr, fk): P2(c0 + 16 cg + 4 q + fk, fr)
+        double *Rw = red[w];
+#pragma unroll
+        for (int cg = 0; cg < 2; cg++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) Rw[(16 * cg + 4 * q + fk) * 16 + fr] = p2[cg][q];
+        if (ABS) {
+#pragma unroll
+            for (int cg = 0; cg < 2; cg++) {
+                double v = a2[cg];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (fk == 0) Rw[CW * 16 + 16 * cg + fr] = v;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < CW * 16 + (ABS ? CW : 0); e += 256) {
+            const double v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+            if (e < CW * 16) {
+                const int cl = e >> 4;
+                if (c0 + cl < NB) P2[(int64_t)(c0 + cl) * 16 + (e & 15)] = v;
+            } else if (c0 + e - CW * 16 < NB) {
+                A2[c0 + e - CW * 16] = v;
+            }
+        }
+        __syncthreads();
+    }
+    // element q of p1[g] in lane (fr, fk): P1(128 w + 16 g + 4 q + fk, fr)
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+        const int r0 = 128 * w + 16 * g;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = r0 + 4 * q + fk;
+            if (r < NB) P1[(int64_t)r * 16 + fr] = p1[g][q];
+        }
+        if (ABS) {
+            double v = a1[g];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (fk == 0 && r0 + fr < NB) A1[r0 + fr] = v;
+        }
+    }
+}
+
+// R (ldr) = B (ldb) - (P1 + P2 sums) for nr right-hand sides; thread = (row,
+// right-hand side), partials summed over n = 0..m, then m' = m..MT-1.
+__global__ __launch_bounds__(256) void k_symm_sum(const double *__restrict__ P, int N, int NB, int MT,
+                                                 const double *__restrict__ B, int ldb, double *__restrict__ R,
+                                                 int ldr, int nr, double *__restrict__ rowabs)
+{
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)N * 16) return;
+    const int row = (int)(e >> 4), j = (int)(e & 15);
+    const int m = row / NB, r = row - m * NB;
+    const int64_t stride = (int64_t)SY_PART * NB;
+    double s = 0.0;
+    for (int n = 0; n <= m; n++) s += P[tile_index(MT, m, n) * stride + (int64_t)r * 16 + j];
+    for (int mm = m; mm < MT; mm++) s += P[tile_index(MT, mm, m) * stride + 16 * (int64_t)NB + (int64_t)r * 16 + j];
+    if (j < nr) R[(int64_t)j * ldr + row] = B[(int64_t)j * ldb + row] - s;
+    if (rowabs && j == 0) {
+        double a = 0.0;
+        for (int n = 0; n <= m; n++) a += P[tile_index(MT, m, n) * stride + 32 * (int64_t)NB + r];
+        for (int mm = m; mm < MT; mm++) a += P[tile_index(MT, mm, m) * stride + 33 * (int64_t)NB + r];
+        rowabs[row] = a;
+    }
+}
+
 // out[j] = max_i |V(i, j)| for the nc columns of V (M x nc, ld); NaN wins.
 // One workgroup per column.
 __global__ __launch_bounds__(256) void k_colamax(const double *__restrict__ V, int ld, int M,
@@ -252,6 +452,7 @@ struct Buf {
     }
 };
 Buf g_l32, g_r32, g_r64, g_w32;
+Buf g_part;      // per-tile partials of the symmetric residual
 
 }  // namespace
 
@@ -262,6 +463,29 @@ int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS
                 const double *dB, int ldb, double *dR, int ldr, double *rowabs)
 {
     const int MT = (N + NB - 1) / NB;
+    if (NB >= 64 && NB <= 512) {     // below 64 the partials outweigh the second read
+        const int64_t ntile = (int64_t)MT * (MT + 1) / 2;
+        double *P = (double *)g_part.get((size_t)ntile * SY_PART * NB * sizeof(double));
+        if (!P) {
+            set_error("out of device memory for the residual partials");
+            return -3;
+        }
+        for (int j0 = 0; j0 < NRHS; j0 += 16) {
+            const int nr = NRHS - j0 < 16 ? NRHS - j0 : 16;
+            double *ra = j0 == 0 ? rowabs : nullptr;
+            const double *Xj = dX + (int64_t)j0 * ldx;
+            if (ra)
+                MI_LAUNCH(c, s, "dsymm_part", 2.0 * N * (double)N * nr, 8.0 * (double)ntile * NB * NB,
+                          k_symm_part<true><<<(unsigned)ntile, 256, 0, s>>>(dA, N, NB, MT, Xj, ldx, nr, P));
+            else
+                MI_LAUNCH(c, s, "dsymm_part", 2.0 * N * (double)N * nr, 8.0 * (double)ntile * NB * NB,
+                          k_symm_part<false><<<(unsigned)ntile, 256, 0, s>>>(dA, N, NB, MT, Xj, ldx, nr, P));
+            MI_LAUNCH(c, s, "dsymm_sum", 0.0, 8.0 * 2.0 * (double)ntile * 16 * NB,
+                      k_symm_sum<<<(unsigned)(((int64_t)N * 16 + 255) / 256), 256, 0, s>>>(
+                          P, N, NB, MT, dB + (int64_t)j0 * ldb, ldb, dR + (int64_t)j0 * ldr, ldr, nr, ra));
+        }
+        return 0;
+    }
     const int strips = (NB + RS_ROWS - 1) / RS_ROWS;
     for (int j0 = 0; j0 < NRHS; j0 += 16) {
         const int nr = NRHS - j0 < 16 ? NRHS - j0 : 16;

@@ -449,16 +449,16 @@ __global__ __launch_bounds__(256) void k_gemv_bwd(const T *__restrict__ Tt, int 
 // One-launch wavefront solve (up to 16 right-hand sides, NB % 32 == 0).
 // Workgroup g (a logical index from an atomic ticket, so every lower index
 // is already running) owns rows [128 g, 128 g + 128) of X, 32 rows per wave.
-// Forward it subtracts L(rows, block h) X(block h) for every block h < g as
-// its flag appears; the L operands of the next block are loaded while the
-// workgroup waits, the X operands all at once after the flag.  Then
-// x_g = W_g y_g with W_g the inverse of the 128 x 128 diagonal triangle
-// (k_inv128, staged in LDS at the start), X is written, every wave fences
-// its stores and the flag is set.  Backward mirrors it
-// from the last block with L^T and W_g^T.  Each L element is read once per
-// sweep; per workgroup the chain costs one block update and one 128-row
-// product.  Every workgroup waits only on lower logical indices, so the
-// launch always drains.
+// Forward it subtracts L(rows, block h) X(block h) for every block h < g - 1
+// as X(block h) appears; the L operands of the next block are loaded while
+// the wave waits.  Then z = W_g y with W_g the inverse of the 128 x 128
+// diagonal triangle (k_inv128, staged in LDS at the start), and, once the
+// last block appears, x_g = z - M_g X(block g - 1) with the link matrix
+// M_g = W_g L(g, g - 1) (k_wave_link), so the chain between two workgroups
+// is one 128 x 128 x 16 product.  Backward mirrors it from the last block
+// with L^T, W_g^T and M_g = W_g^T L(g + 1, g)^T.  Each L element is read
+// once per sweep.  Every workgroup waits only on lower logical indices, so
+// the launch always drains.
 constexpr int WV_RB = 128;
 constexpr int WV_WS = 136;   // LDS column stride of W_g (128 + 8: fragment reads in two passes)
 
@@ -469,23 +469,45 @@ __device__ inline const T *sym_elem(const T *Tt, int MT, int NB, int row, int co
     return Tt + tile_index(MT, m, n) * (int64_t)NB * NB + (int64_t)(col - n * NB) * NB + (row - m * NB);
 }
 
-// Hand-off between workgroups (message passing through uncached memory):
-// the flags and the staging copy xs of X live in uncached device memory, so
-// every load and store reaches memory; the producer waits for its xs stores
-// to complete before it stores its flag, and the consumer issues its xs
-// loads only after its flag load returned the set value.  (Release/acquire
-// fences on cached memory would write back and invalidate the whole L2 at
-// every hand-off instead.)  xs holds block g's 128 rows x 16 columns at
-// xs + g * 2048, row-major by column index: xs[g * 2048 + j * 128 + i].
+// Hand-off between workgroups: the staging copy xs of X lives in uncached
+// device memory (every access reaches memory, so values stored on one XCD
+// are visible on the others without cache maintenance) and is filled with a
+// signalling NaN before the launch.  Arithmetic never produces a signalling
+// NaN, so a staged value that differs from it is final: the producer stores
+// its x values (single-copy atomic, one element per store) and the consumer
+// polls the values themselves, with no separate flag and no wait for store
+// completion.  Block g occupies xs + g * 2048 in the order given by wv_xs,
+// so that every load instruction of a consumer covers whole cache lines.
+template <class T> struct WvSent;
+template <> struct WvSent<double> {
+    typedef unsigned long long U;
+    static constexpr U v = 0x7FF4A5A5A5A5A5A5ull;
+};
+template <> struct WvSent<float> {
+    typedef unsigned int U;
+    static constexpr U v = 0x7FA5A5A5u;
+};
 
-__device__ inline void wave_flag_wait(int *flags, int h)
+template <class T>
+__device__ inline typename WvSent<T>::U wv_load(const T *p)
 {
-    if (threadIdx.x == 0) {
-        int *fp = flags + h;
-        asm volatile("" : "+v"(fp));        // vector-memory access
-        while (__hip_atomic_load(fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) __builtin_amdgcn_s_sleep(1);
-    }
-    __syncthreads();
+    typedef typename WvSent<T>::U U;
+    const U *q = (const U *)p;
+    asm volatile("" : "+v"(q));             // vector-memory access
+    return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// a value equal to the sentinel (only possible if a signalling NaN came in
+// unquieted) is stored as a quiet NaN, so that every wait ends
+template <class T>
+__device__ inline void wv_store(T *p, T v)
+{
+    typedef typename WvSent<T>::U U;
+    U *q = (U *)p;
+    asm volatile("" : "+v"(q));
+    U b = __builtin_bit_cast(U, v);
+    if (b == WvSent<T>::v) b |= (U)1 << (sizeof(U) == 8 ? 51 : 22);
+    __hip_atomic_store(q, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Inverse of each 128 x 128 diagonal triangle of L (column-major, 128 x 128
@@ -542,13 +564,83 @@ __global__ __launch_bounds__(256) void k_inv128(const T *__restrict__ Tt, int N,
     }
 }
 
+// Link matrices M_g = op(W_g) op(L)(g, h), h the block solved just before g
+// (g - 1 forward, g + 1 backward): 128 x 128 at M + g * 16384, column-major
+// forward and row-major backward (the order in which the solve reads it).
+// op(L)(g, h)(j, k) = L(128 g + j, 128 h + k) forward, L(128 h + k, 128 g + j)
+// backward; rows beyond N are zero.  One workgroup per block; thread (k =
+// tid % 128) accumulates rows i = tid / 128 + 2 e of column k.
+template <class T, bool TRANS>
+__global__ __launch_bounds__(256) void k_wave_link(const T *__restrict__ Tt, int N, int NB, int MT,
+                                                  const T *__restrict__ W128, T *__restrict__ M)
+{
+    __shared__ T sw[128][33];     // op(W_g)(i, j0 + jj)
+    __shared__ T sl[32][129];     // op(L)(j0 + jj, k)
+    const int g = blockIdx.x, nwg = gridDim.x, tid = threadIdx.x;
+    if (TRANS ? g + 1 >= nwg : g == 0) return;
+    const int h = TRANS ? g + 1 : g - 1;
+    const T *Wg = W128 + (int64_t)g * 16384;
+    const int k = tid & 127, i0 = tid >> 7;
+    T acc[64];
+#pragma unroll
+    for (int e = 0; e < 64; e++) acc[e] = T(0);
+    for (int j0 = 0; j0 < 128; j0 += 32) {
+        for (int e = tid; e < 128 * 32; e += 256) {
+            const int i = e & 127, jj = e >> 7;
+            sw[i][jj] = TRANS ? Wg[i * 128 + j0 + jj] : Wg[(j0 + jj) * 128 + i];
+        }
+        for (int e = tid; e < 32 * 128; e += 256) {
+            const int jj = e & 31, kk = e >> 5;
+            const int row = TRANS ? 128 * h + kk : 128 * g + j0 + jj;
+            const int col = TRANS ? 128 * g + j0 + jj : 128 * h + kk;
+            sl[jj][kk] = row < N ? *sym_elem(Tt, MT, NB, row, col) : T(0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int jj = 0; jj < 32; jj++) {
+            const T l = sl[jj][k];
+#pragma unroll
+            for (int e = 0; e < 64; e++) acc[e] += sw[i0 + 2 * e][jj] * l;
+        }
+        __syncthreads();
+    }
+    T *Mg = M + (int64_t)g * 16384;            // column-major forward, row-major backward
+#pragma unroll
+    for (int e = 0; e < 64; e++) Mg[TRANS ? (i0 + 2 * e) * 128 + k : k * 128 + i0 + 2 * e] = acc[e];
+}
+
+// Index mappings of the wavefront solve.  A wave owns 32 rows of its block;
+// MFMA row i (0..15) of product t (0, 1) is row wv_row(t, i) of them, and
+// element kq (0..7) of lane group f (0..3) of a 32-column slice of the
+// contraction is column wv_k(kq, f) of it.  Forward, two consecutive rows
+// belong to one lane (2 i + t), so that the column-major L operands load as
+// pairs; backward, four consecutive columns belong to one lane, so that the
+// rows of L (the columns of L^T) load as runs of four.
+template <bool TRANS> __device__ inline int wv_row(int t, int i) { return TRANS ? 16 * t + i : 2 * i + t; }
+template <bool TRANS> __device__ inline int wv_k(int kq, int f) { return TRANS ? 16 * (kq >> 2) + 4 * f + (kq & 3) : 4 * kq + f; }
+
+// Staging position of X(row R of the block, right-hand side j).  FP64: [row]
+// [rhs], read one element per lane and load.  FP32: consumer order, two
+// values per 64-bit load: the element kq of lane (j, f) of slice hb sits at
+// ((hb * 4 + kq / 2) * 64 + 16 f + j) * 2 + kq % 2.
+template <class T, bool TRANS> __device__ inline int wv_xs(int R, int j)
+{
+    if (sizeof(T) == 8) return R * RHSB + j;
+    const int hb = R >> 5, r = R & 31;
+    const int kq = TRANS ? 4 * (r >> 4) + (r & 3) : r >> 2;
+    const int f = TRANS ? (r >> 2) & 3 : r & 3;
+    return ((hb * 4 + (kq >> 1)) * 64 + 16 * f + j) * 2 + (kq & 1);
+}
+
 template <class T, bool TRANS>
 __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, int N, int NB, int MT,
-                                                     const T *__restrict__ W128, T *__restrict__ B, int ldb,
-                                                     int nr, T *__restrict__ xs, int *__restrict__ flags,
+                                                     const T *__restrict__ W128, const T *__restrict__ Mlink,
+                                                     T *__restrict__ B, int ldb, int nr, T *__restrict__ xs,
                                                      int *__restrict__ ticket)
 {
     typedef typename MF<T>::v4 V4;
+    typedef typename MF<T>::v2 V2;
+    typedef typename WvSent<T>::U U;
     __shared__ T Ws[128 * WV_WS];    // a-operands of the diagonal product: Ws[k * WV_WS + i] = op(W)(i, k)
     __shared__ T Ys[128][17];
     __shared__ int s_g;
@@ -589,12 +681,13 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
     for (int t = 0; t < 2; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int row = rw + 16 * t + MF<T>::row(q, fk);
+            const int row = rw + wv_row<TRANS>(t, MF<T>::row(q, fk));
             const T v = B[(int64_t)frc * ldb + min(row, N - 1)];
             acc[t][q] = (row < N && fr < nr) ? v : T(0);
         }
 
-    // a-operands of one other block h: op(L)(my 32 rows, the block's 128 columns)
+    // a-operands of one other block h: op(L)(my row wv_row(t, fr), column
+    // 32 hb + wv_k(kq, fk) of the block)
     T la[4][8][2];
     auto load_a = [&](int h) {
 #pragma unroll
@@ -602,49 +695,133 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
             const int c0 = h * WV_RB + 32 * hb;
             const int cc = min(c0, N - 1) & ~31;     // a valid 32-aligned block when c0 >= N
             const int rr = min(rw, N - 1) & ~31;
-            const T *pA = TRANS ? sym_elem(Tt, MT, NB, cc, rr) : sym_elem(Tt, MT, NB, rr, cc);
+            if (!TRANS) {
+                const T *pA = sym_elem(Tt, MT, NB, rr, cc) + 2 * fr;
 #pragma unroll
-            for (int kq = 0; kq < 8; kq++) {
-                const int kk = 4 * kq + fk;
+                for (int kq = 0; kq < 8; kq++) {
+                    const int kk = 4 * kq + fk;
+                    const V2 v = *(const V2 *)(pA + (int64_t)kk * NB);
+                    const bool ck = c0 < N && c0 + kk < N;
+                    la[hb][kq][0] = (ck && rw + 2 * fr < N) ? v.x : T(0);
+                    la[hb][kq][1] = (ck && rw + 2 * fr + 1 < N) ? v.y : T(0);
+                }
+            } else {
+                const T *pA = sym_elem(Tt, MT, NB, cc, rr) + 4 * fk;
 #pragma unroll
                 for (int t = 0; t < 2; t++) {
                     const int i = 16 * t + fr;
-                    const T v = TRANS ? pA[(int64_t)i * NB + kk] : pA[(int64_t)kk * NB + i];
-                    la[hb][kq][t] = (c0 < N && rw + i < N && c0 + kk < N) ? v : T(0);
+#pragma unroll
+                    for (int kg = 0; kg < 2; kg++) {
+                        const T *p = pA + (int64_t)i * NB + 16 * kg;
+                        const V2 v0 = *(const V2 *)p, v1 = *(const V2 *)(p + 2);
+                        const T v[4] = {v0.x, v0.y, v1.x, v1.y};
+#pragma unroll
+                        for (int e = 0; e < 4; e++) {
+                            const int kk = 16 * kg + 4 * fk + e;
+                            la[hb][4 * kg + e][t] = (c0 < N && rw + i < N && c0 + kk < N) ? v[e] : T(0);
+                        }
+                    }
                 }
             }
         }
     };
-    const int nprev = TRANS ? nwg - 1 - g : g;
-    if (nprev > 0 && rw < N) load_a(TRANS ? nwg - 1 : 0);
-    for (int s = 0; s < nprev; s++) {
-        const int h = TRANS ? nwg - 1 - s : s;
-        wave_flag_wait(flags, h);
-        if (rw < N) {
-            T bx[4][8];                           // -X(block h): the update subtracts
+    // a-operands of the link matrix: M_g(32 w + wv_row(t, fr), 32 hb + wv_k(kq, fk)),
+    // column-major forward, row-major backward
+    auto load_m = [&]() {
+        const T *Mg = Mlink + (int64_t)g * 16384;
 #pragma unroll
-            for (int hb = 0; hb < 4; hb++)
+        for (int hb = 0; hb < 4; hb++) {
+            if (!TRANS) {
 #pragma unroll
                 for (int kq = 0; kq < 8; kq++) {
-                    const int r = h * WV_RB + 32 * hb + 4 * kq + fk;
-                    const T v = xs[(int64_t)h * (WV_RB * RHSB) + fr * WV_RB + 32 * hb + 4 * kq + fk];
-                    bx[hb][kq] = (fr < nr && r < N) ? -v : T(0);
+                    const V2 v = *(const V2 *)(Mg + (32 * hb + 4 * kq + fk) * 128 + 32 * wave + 2 * fr);
+                    la[hb][kq][0] = v.x;
+                    la[hb][kq][1] = v.y;
                 }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 2; t++)
+#pragma unroll
+                    for (int kg = 0; kg < 2; kg++) {
+                        const T *p = Mg + (32 * wave + 16 * t + fr) * 128 + 32 * hb + 16 * kg + 4 * fk;
+                        const V2 v0 = *(const V2 *)p, v1 = *(const V2 *)(p + 2);
+                        la[hb][4 * kg + 0][t] = v0.x;
+                        la[hb][4 * kg + 1][t] = v0.y;
+                        la[hb][4 * kg + 2][t] = v1.x;
+                        la[hb][4 * kg + 3][t] = v1.y;
+                    }
+            }
+        }
+    };
+    // -X(block h) as b-operands, once every value has been stored: for the
+    // last block (fin) lanes 0..3 first poll one value of each producer
+    // wave; then the wave loads its values, and loads again while any of
+    // them is still the sentinel
+    auto wait_x = [&](int h, bool fin, T (&bx)[4][8]) {
+        const T *xh = xs + (int64_t)h * (WV_RB * RHSB);
+        if (fin && lane < 4)
+            while (wv_load(xh + wv_xs<T, TRANS>(32 * lane + 31, RHSB - 1)) == WvSent<T>::v)
+                __builtin_amdgcn_s_sleep(1);
+        for (;;) {
+            int ok = 1;
+            if constexpr (sizeof(T) == 8) {
+#pragma unroll
+                for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+                    for (int kq = 0; kq < 8; kq++) {
+                        const U b = wv_load(xh + (32 * hb + wv_k<TRANS>(kq, fk)) * RHSB + fr);
+                        ok &= (b != WvSent<T>::v);
+                        bx[hb][kq] = __builtin_bit_cast(T, b);
+                    }
+            } else {
+#pragma unroll
+                for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+                    for (int kp = 0; kp < 4; kp++) {
+                        const unsigned long long *p =
+                            (const unsigned long long *)(xh + ((hb * 4 + kp) * 64 + lane) * 2);
+                        asm volatile("" : "+v"(p));
+                        const unsigned long long b = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+                        ok &= (lo != (unsigned)WvSent<float>::v) & (hi != (unsigned)WvSent<float>::v);
+                        bx[hb][2 * kp] = __builtin_bit_cast(T, lo);
+                        bx[hb][2 * kp + 1] = __builtin_bit_cast(T, hi);
+                    }
+            }
+            if (__all(ok)) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+            for (int kq = 0; kq < 8; kq++) {
+                const int r = h * WV_RB + 32 * hb + wv_k<TRANS>(kq, fk);
+                bx[hb][kq] = (fr < nr && r < N) ? -bx[hb][kq] : T(0);
+            }
+    };
+    const int nprev = TRANS ? nwg - 1 - g : g;
+    if (nprev > 1 && rw < N) load_a(TRANS ? nwg - 1 : 0);
+    for (int s = 0; s + 1 < nprev; s++) {
+        const int h = TRANS ? nwg - 1 - s : s;
+        if (rw < N) {
+            T bx[4][8];
+            wait_x(h, false, bx);
 #pragma unroll
             for (int hb = 0; hb < 4; hb++)
 #pragma unroll
                 for (int kq = 0; kq < 8; kq++)
 #pragma unroll
                     for (int t = 0; t < 2; t++) acc[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], acc[t]);
-            if (s + 1 < nprev) load_a(TRANS ? nwg - 2 - s : s + 1);
+            if (s + 2 < nprev) load_a(TRANS ? nwg - 2 - s : s + 1);
         }
     }
+    if (nprev > 0 && rw < N) load_m();
 
-    // x_g = op(W_g) y_g
+    // z = op(W_g) y
 #pragma unroll
     for (int t = 0; t < 2; t++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) Ys[32 * wave + 16 * t + MF<T>::row(q, fk)][fr] = acc[t][q];
+        for (int q = 0; q < 4; q++) Ys[32 * wave + wv_row<TRANS>(t, MF<T>::row(q, fk))][fr] = acc[t][q];
     __syncthreads();
     V4 x[2];
 #pragma unroll
@@ -654,29 +831,44 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
         const int kk = 4 * kq + fk;
         const T b = Ys[kk][fr];
 #pragma unroll
-        for (int t = 0; t < 2; t++) x[t] = MF<T>::mfma(Ws[kk * WV_WS + 32 * wave + 16 * t + fr], b, x[t]);
+        for (int t = 0; t < 2; t++)
+            x[t] = MF<T>::mfma(Ws[kk * WV_WS + 32 * wave + wv_row<TRANS>(t, fr)], b, x[t]);
     }
+    // x_g = z - M_g X(last block)
+    if (nprev > 0 && rw < N) {
+        T bx[4][8];
+        wait_x(TRANS ? g + 1 : g - 1, true, bx);
+#pragma unroll
+        for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+            for (int kq = 0; kq < 8; kq++)
+#pragma unroll
+                for (int t = 0; t < 2; t++) x[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], x[t]);
+    }
+    T *xg = xs + (int64_t)g * (WV_RB * RHSB);
 #pragma unroll
     for (int t = 0; t < 2; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int row = rw + 16 * t + MF<T>::row(q, fk);
-            xs[(int64_t)g * (WV_RB * RHSB) + fr * WV_RB + 32 * wave + 16 * t + MF<T>::row(q, fk)] = x[t][q];
-            if (row < N && fr < nr) B[(int64_t)fr * ldb + row] = x[t][q];
+            const int rb = 32 * wave + wv_row<TRANS>(t, MF<T>::row(q, fk));
+            wv_store(xg + wv_xs<T, TRANS>(rb, fr), x[t][q]);
+            if (r0 + rb < N && fr < nr) B[(int64_t)fr * ldb + r0 + rb] = x[t][q];
         }
-    // every wave's X stores complete, then the flag
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (tid == 0) {
-        int *fp = flags + g;
-        asm volatile("" : "+v"(fp));
-        __hip_atomic_store(fp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
-__global__ void k_wave_reset(int *__restrict__ flags, int n)
+// Before each wavefront launch: staging filled with the sentinel, ticket 0.
+template <class T>
+__global__ void k_wave_reset(T *__restrict__ xs, int64_t n, int *__restrict__ ticket)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) flags[i] = 0;
+    typedef typename WvSent<T>::U U;
+    U *q = (U *)xs;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        q[i] = WvSent<T>::v;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int *tp = ticket;
+        asm volatile("" : "+v"(tp));          // vector store
+        *tp = 0;
+    }
 }
 
 template <class T>
@@ -706,8 +898,8 @@ int ensure_sws(size_t bytes)
     return 0;
 }
 
-// Hand-off memory of the wavefront solve: the flags (and ticket) and a
-// staging copy of X, in uncached device memory so that every access reaches
+// Hand-off memory of the wavefront solve: the ticket and the staging copy
+// of X, in uncached device memory so that every access reaches
 // memory and a value stored by one XCD is visible to the others without cache
 // maintenance.  Uncached allocations are never freed once a kernel has read
 // them (a grown buffer replaces the old one, which stays allocated).
@@ -733,11 +925,11 @@ int set_lds(const void *f, size_t bytes)
 template <class T> struct SN;
 template <> struct SN<double> {
     static constexpr const char *diag = "dtrsm_diag", *gemv = "dtrsm_gemv", *inv = "dtrsm_inv_diag",
-                                *scale = "dtrsm_scale", *wave = "dtrsm_wave";
+                                *scale = "dtrsm_scale", *wave = "dtrsm_wave", *link = "dtrsm_link";
 };
 template <> struct SN<float> {
     static constexpr const char *diag = "strsm_diag", *gemv = "strsm_gemv", *inv = "strsm_inv_diag",
-                                *scale = "strsm_scale", *wave = "strsm_wave";
+                                *scale = "strsm_scale", *wave = "strsm_wave", *link = "strsm_link";
 };
 
 template <class T, int NR>
@@ -812,8 +1004,9 @@ size_t trsm_ws_elems(int N, int NB)
 {
     const int MT = (N + NB - 1) / NB;
     const size_t w32 = (size_t)MT * ((NB + 31) / 32) * 1024;
-    // plus the 128 x 128 inverses of the wavefront solve
-    return w32 + (NB % 32 == 0 ? (size_t)((N + WV_RB - 1) / WV_RB) * 16384 : 0);
+    // plus the 128 x 128 inverses and the forward and backward link matrices
+    // of the wavefront solve
+    return w32 + (NB % 32 == 0 ? (size_t)((N + WV_RB - 1) / WV_RB) * 3 * 16384 : 0);
 }
 
 // Inverses of the 32x32 diagonal blocks of every diagonal tile into W
@@ -830,6 +1023,11 @@ int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int NB, const T *dT, T 
         T *W128 = W + (size_t)MT * nblk * 1024;
         MI_LAUNCH(c, s, SN<T>::inv, (double)nwg * 128 * 128 * 128 / 3.0, (double)nwg * 16384 * 2.0 * sizeof(T),
                   k_inv128<T><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W, W128));
+        T *Mf = W128 + (size_t)nwg * 16384, *Mb = Mf + (size_t)nwg * 16384;
+        MI_LAUNCH(c, s, SN<T>::link, (double)nwg * 128 * 128 * 128, (double)nwg * 16384 * 3.0 * sizeof(T),
+                  k_wave_link<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mf));
+        MI_LAUNCH(c, s, SN<T>::link, (double)nwg * 128 * 128 * 128, (double)nwg * 16384 * 3.0 * sizeof(T),
+                  k_wave_link<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mb));
     }
     return 0;
 }
@@ -844,21 +1042,21 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
         // one wavefront launch per chunk of 16 right-hand sides
         const int nwg = (N + WV_RB - 1) / WV_RB;
         int rc;
-        if ((rc = ensure_uncached(g_wflags, (size_t)(nwg + 1) * sizeof(int)))) return rc;
+        if ((rc = ensure_uncached(g_wflags, sizeof(int)))) return rc;
         if ((rc = ensure_uncached(g_wstage, (size_t)nwg * WV_RB * RHSB * sizeof(T)))) return rc;
-        int *flags = (int *)g_wflags.w;
+        int *ticket = (int *)g_wflags.w;
         T *xs = (T *)g_wstage.w;
         const T *W128 = W + (size_t)MT * ((NB + 31) / 32) * 1024;
+        const T *Mf = W128 + (size_t)nwg * 16384, *Mb = Mf + (size_t)nwg * 16384;
+        const int64_t nst = (int64_t)nwg * WV_RB * RHSB;
         for (int j0 = 0; j0 < NRHS; j0 += RHSB) {
             const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
             T *Bj = dB + (int64_t)j0 * ldb;
-            k_wave_reset<<<1, 256, 0, s>>>(flags, nwg);
+            k_wave_reset<T><<<(unsigned)std::min<int64_t>((nst + 255) / 256, 1024), 256, 0, s>>>(xs, nst, ticket);
             MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
                       (trans == 111
-                       ? k_trsm_wave<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Bj, ldb, nr, xs, flags,
-                                                                  flags + nwg)
-                       : k_trsm_wave<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Bj, ldb, nr, xs, flags,
-                                                                 flags + nwg)));
+                       ? k_trsm_wave<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs, ticket)
+                       : k_trsm_wave<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs, ticket)));
         }
         return 0;
     }
