@@ -95,7 +95,7 @@ __device__ inline void mfma_panel_2(const T *__restrict__ X, int ldx, int r0, in
 // and the loads and stores of the column-major C are 128-byte runs.
 template <class T>
 __device__ inline void syrk_unit32(T *__restrict__ A, int ld, int n, const T *__restrict__ P, int RC, int r0,
-                                   int c0, int fr, int fk)
+                                   int c0, int fr, int fk, T *__restrict__ dsm = nullptr)
 {
     typedef typename MF<T>::v4 V4;
     V4 acc[2][2];                                    // acc[i][j]: rows r0+16i.., columns c0+16j..
@@ -128,12 +128,17 @@ __device__ inline void syrk_unit32(T *__restrict__ A, int ld, int n, const T *__
             for (int q = 0; q < 4; q++) {
                 const int r = r0 + 16 * i + fr, c = c0 + 16 * j + MF<T>::row(q, fk);
                 if (r < n && c < n && r >= c) A[(int64_t)c * ld + r] = acc[i][j][q];
+                // the unit also to LDS, column-major with stride 33 (its lower
+                // part is what the diagonal factor reads)
+                if (dsm) dsm[(c - c0) * 33 + (r - r0)] = acc[i][j][q];
             }
 }
 
 // POTRF(k): right-looking by 32-column blocks in one workgroup of 8 waves,
 // with look-ahead inside the tile.  LDS holds the current block column X
-// (rows j0..n-1, 32 columns) as P[c * RC + row] (absolute rows) and W_jj.
+// (rows j0..n-1, 32 columns) as P[c * RC + row] (absolute rows), W_jj, and
+// the next diagonal block as its update left it (wave 0 updates that block
+// itself and reads it back from LDS rather than from the tile).
 // Per block j0:
 //   wave 0     L_jj = chol(D), W_jj = L_jj^{-1} in one register sweep over the
 //              32 columns (lanes 0..31: rows of L, lanes 32..63: columns of
@@ -159,6 +164,7 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
     typedef typename MF<T>::v4 V4;
     T *P = reinterpret_cast<T *>(smem_raw);          // 32 columns x RC rows (RC odd, >= n + 1)
     T *Ws = P + 32 * RC;                             // W(i, j) at Ws[j*32 + i]
+    T *Dsm = Ws + 1024;                              // next diagonal block, column-major, stride 33
     __shared__ int s_fail;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
@@ -168,16 +174,20 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
     // v[c]: lanes < 32 hold D(lane, c) of the current diagonal block, lanes
     // >= 32 the identity column that becomes W(:, lane - 32)
     T v[32];
-    auto load_diag = [&](int j0) {
+    // from_lds: the block as the next-column update left it in Dsm (lower
+    // part; the strict upper part of v is never read)
+    auto load_diag = [&](int j0, bool from_lds) {
         int j = lane - 32;
         asm volatile("" : "+v"(j));                  // built per block: keeps the identity out of LICM
+        const int lr = min(lane, 31);
 #pragma unroll
         for (int c = 0; c < 32; c++) {
             const int r = j0 + lane, cc = j0 + c;
-            v[c] = lrow ? ((r < n && cc < n) ? A[(int64_t)cc * ld + r] : T(0)) : (c == j ? T(1) : T(0));
+            const T a = from_lds ? Dsm[c * 33 + lr] : ((r < n && cc < n) ? A[(int64_t)cc * ld + r] : T(0));
+            v[c] = lrow ? ((r < n && cc < n) ? a : T(0)) : (c == j ? T(1) : T(0));
         }
     };
-    if (wave == 0) load_diag(0);
+    if (wave == 0) load_diag(0, false);
     __syncthreads();
 
     for (int j0 = 0; j0 < n; j0 += 32) {
@@ -255,10 +265,10 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
         const int b1 = j0 + 32;
         if (b1 < n) {
             for (int u = wave; u < (n - b1 + 31) / 32; u += 8)
-                syrk_unit32<T>(A, ld, n, P, RC, b1 + 32 * u, b1, fr, fk);
+                syrk_unit32<T>(A, ld, n, P, RC, b1 + 32 * u, b1, fr, fk, u == 0 ? Dsm : nullptr);
         }
         __syncthreads();
-        if (wave == 0 && b1 < n) load_diag(b1);
+        if (wave == 0 && b1 < n) load_diag(b1, true);
     }
     if (tid == 0 && s_fail) atomicCAS(d_info, 0, info_base + s_fail);
 }
@@ -1070,7 +1080,7 @@ int potrf_lds_bytes(int NB, int *RC)
 {
     if (!valu_panel<T>()) {
         *RC = ((NB + 15) & ~15) + 1;
-        return (32 * (*RC) + 1024) * (int)sizeof(T);
+        return (32 * (*RC) + 1024 + 32 * 33) * (int)sizeof(T);
     }
     *RC = ((NB + 3) & ~3) + 4;
     return (PJB * (*RC) + 1024) * (int)sizeof(double);
