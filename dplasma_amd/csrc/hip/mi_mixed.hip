@@ -16,7 +16,7 @@
 //
 // Kernels here: FP64 <-> FP32 rounding (tiles and column-major blocks),
 // the FP64 symmetric residual R = B - A X over the lower tiles, and column
-// infinity norms.  The residual is deterministic.  For 64 <= NB <= 512 every
+// infinity norms.  The residual is deterministic.  For even 64 <= NB <= 512 every
 // stored A element is read once: one workgroup per tile forms A_mn X_n and
 // A_mn^T X_m on FP64 MFMA into per-tile partials, and a second pass sums the
 // partials of each row in a fixed order.  Other tiles use a strip kernel
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void k_dsymm_resid(const double *__restrict__ 
     }
 }
 
-// The same residual with every stored element of A read once (64 <= NB <= 512),
+// The same residual with every stored element of A read once (64 <= NB <= 512, NB even),
 // in two launches.  k_symm_part: one workgroup per stored tile (m, n),
 // m >= n, computes on FP64 MFMA (16x16x4, the 16 columns are the right-hand
 // sides)
@@ -206,9 +206,9 @@ __global__ __launch_bounds__(256) void k_dsymm_resid(const double *__restrict__ 
 // tile only the lower triangle is stored: P1 takes it with the diagonal,
 // P2 the strict lower part, so P1 + P2 is the full symmetric product.  Wave
 // w owns rows [128 w, 128 w + 128) of the tile for P1.  Per 32-column chunk
-// each wave reads its 16 x 32 A blocks coalesced (MFMA a-operand layout:
-// lane 16 k + i holds A(i, k)); the a-operands of P2 (rows along k) are the
-// same values transposed through a per-wave LDS buffer.  The four waves'
+// each wave reads its 32 x 32 A blocks as pairs of consecutive rows (MFMA
+// row i of product t is row 2 i + t); the a-operands of P2 (rows along k)
+// are the same values transposed through a per-wave LDS buffer.  The four waves'
 // P2 contributions to a chunk are summed in LDS in a fixed order.
 // k_symm_sum then forms R = B - sum_n P1(m, n) - sum_m' P2(m', m) for each
 // row, in a fixed order, so the result is deterministic.
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256, 2) void k_symm_part(const double *__restrict__
                                                      double *__restrict__ P)
 {
     constexpr int CW = 32;                       // columns per chunk
-    __shared__ double Tb[4][CW * SY_TS];
+    __shared__ double Tb[4][CW * 33];            // per wave: 32 x 32 block of A, [column][row], stride 33
     __shared__ double red[4][CW * 16 + CW];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
@@ -243,42 +243,51 @@ __global__ __launch_bounds__(256, 2) void k_symm_part(const double *__restrict__
     const double *Xn = X + (int64_t)jc * ldx + (int64_t)n * NB;
     const double *Xm = X + (int64_t)jc * ldx + (int64_t)m * NB;
     double *T = Tb[w];
-    const int ng = min(8, max(0, (rows_m - 128 * w + 15) / 16));   // row groups of this wave (uniform)
+    // row groups of 32 (two MFMA row groups t, MFMA row i at row 2 i + t, so
+    // that a lane loads two consecutive rows at once)
+    const int ng = min(4, max(0, (rows_m - 128 * w + 31) / 32));   // uniform per wave
 
-    dx4 p1[8];
-    double a1[8];
+    dx4 p1[4][2];
+    double a1[4][2];
 #pragma unroll
-    for (int g = 0; g < 8; g++) {
-        p1[g] = dx4{0.0, 0.0, 0.0, 0.0};
-        a1[g] = 0.0;
-    }
-    // operands of row group g of chunk c0: a-operands of P1 A(128 w + 16 g + fr,
-    // c0 + 4 u + fk) (32-bit offsets from the tile base) and b-operands of P2
-    // X_m(128 w + 16 g + 4 s + fk, fr).  The row is made opaque per call, so
-    // that row-dependent addresses are not precomputed for all row groups.
-    auto load_grp = [&](double (&a)[8], double (&xm)[4], int c0, int g) {
-        int rb = 128 * w + 16 * g;
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++) {
+            p1[g][tt] = dx4{0.0, 0.0, 0.0, 0.0};
+            a1[g][tt] = 0.0;
+        }
+    // a-operands of row group g of chunk c0: A(rb + 2 fr + tt, c0 + 4 u + fk) as
+    // pairs (the tile is column-major, NB even); the row base is made opaque
+    // per call, so that row-dependent addresses are not precomputed for all
+    // row groups
+    auto load_a = [&](double2 (&a)[8], int c0, int g) {
+        int rb = 128 * w + 32 * g;
         asm volatile("" : "+v"(rb));
-        const int r = rb + fr;
-        const unsigned rc = (unsigned)min(r, rows_m - 1);
+        const int r = rb + 2 * fr;
+        const unsigned rc = (unsigned)min(r, ((rows_m - 1) & ~1));
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int c = c0 + 4 * u + fk;
-            const double v = At[(unsigned)min(c, cols_n - 1) * (unsigned)NB + rc];
+            const double2 v = *(const double2 *)(At + (unsigned)min(c, cols_n - 1) * (unsigned)NB + rc);
             // in the tile, and in the lower triangle on the diagonal tile
-            const int ok = (rows_m - 1 - r) | (cols_n - 1 - c) | (dg ? r - c : 0);
-            a[u] = ok >= 0 ? v : 0.0;
+            const int ok0 = (rows_m - 1 - r) | (cols_n - 1 - c) | (dg ? r - c : 0);
+            const int ok1 = (rows_m - 2 - r) | (cols_n - 1 - c) | (dg ? r + 1 - c : 0);
+            a[u].x = ok0 >= 0 ? v.x : 0.0;
+            a[u].y = ok1 >= 0 ? v.y : 0.0;
         }
+    };
+    // b-operands of P2 for row group g: X_m(rb + 4 s + fk, fr)
+    auto load_xm = [&](double (&xm)[8], int g) {
+        int rb = 128 * w + 32 * g;
+        asm volatile("" : "+v"(rb));
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
+        for (int s = 0; s < 8; s++) {
             const int rr = rb + 4 * s + fk;
             const double v = Xm[(unsigned)min(rr, rows_m - 1)];
             xm[s] = (jok && rr < rows_m) ? v : 0.0;
         }
     };
     const int nch = (NB + CW - 1) / CW;
-    double a[8], xm[4];
-    if (ng > 0) load_grp(a, xm, 0, 0);
     for (int ch = 0; ch < nch; ch++) {
         const int c0 = CW * ch;
         // b-operands of P1: X_n(c0 + 4 u + fk, fr)
@@ -297,35 +306,37 @@ __global__ __launch_bounds__(256, 2) void k_symm_part(const double *__restrict__
             a2[cg] = 0.0;
         }
 #pragma unroll
-        for (int g = 0; g < 8; g++) {
+        for (int g = 0; g < 4; g++) {
             if (g < ng) {                         // wave-uniform
-                // the next row group's operands (across the chunk boundary too)
-                // are in flight while this one is used
+                // this group's operands (the second workgroup on the CU hides
+                // their latency)
                 asm volatile("" ::: "memory");
-                double an[8], xmn[4];
-                if (g + 1 < ng) load_grp(an, xmn, c0, g + 1);
-                else if (ch + 1 < nch) load_grp(an, xmn, c0 + CW, 0);
-                const int r0 = 128 * w + 16 * g;
+                double2 a[8];
+                load_a(a, c0, g);
+                double xm[8];
+                load_xm(xm, g);
+                const int r0 = 128 * w + 32 * g;
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
-                    p1[g] = MF<double>::mfma(a[u], xn[u], p1[g]);
-                    if (ABS) a1[g] += fabs(a[u]);
-                    T[(4 * u + fk) * SY_TS + fr] = a[u];
+                    p1[g][0] = MF<double>::mfma(a[u].x, xn[u], p1[g][0]);
+                    p1[g][1] = MF<double>::mfma(a[u].y, xn[u], p1[g][1]);
+                    if (ABS) {
+                        a1[g][0] += fabs(a[u].x);
+                        a1[g][1] += fabs(a[u].y);
+                    }
+                    T[(4 * u + fk) * 33 + 2 * fr] = a[u].x;
+                    T[(4 * u + fk) * 33 + 2 * fr + 1] = a[u].y;
                 }
 #pragma unroll
                 for (int cg = 0; cg < 2; cg++)
 #pragma unroll
-                    for (int s = 0; s < 4; s++) {
+                    for (int s = 0; s < 8; s++) {
                         // A(r0 + 4 s + fk, c0 + 16 cg + fr); strict lower part on the diagonal tile
-                        double v = T[(16 * cg + fr) * SY_TS + 4 * s + fk];
+                        double v = T[(16 * cg + fr) * 33 + 4 * s + fk];
                         if (dg && !(r0 + 4 * s + fk > c0 + 16 * cg + fr)) v = 0.0;
                         p2[cg] = MF<double>::mfma(v, xm[s], p2[cg]);
                         if (ABS) a2[cg] += fabs(v);
                     }
-#pragma unroll
-                for (int u = 0; u < 8; u++) a[u] = an[u];
-#pragma unroll
-                for (int s = 0; s < 4; s++) xm[s] = xmn[s];
             }
         }
         // element q of p2[cg] in lane (fr, fk): P2(c0 + 16 cg + 4 q + fk, fr)
@@ -355,20 +366,23 @@ __global__ __launch_bounds__(256, 2) void k_symm_part(const double *__restrict__
         }
         __syncthreads();
     }
-    // element q of p1[g] in lane (fr, fk): P1(128 w + 16 g + 4 q + fk, fr)
+    // element q of p1[g][tt] in lane (fr, fk): P1(128 w + 32 g + 2 (4 q + fk) + tt, fr)
 #pragma unroll
-    for (int g = 0; g < 8; g++) {
-        const int r0 = 128 * w + 16 * g;
+    for (int g = 0; g < 4; g++) {
+        const int r0 = 128 * w + 32 * g;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int r = r0 + 4 * q + fk;
-            if (r < NB) P1[(int64_t)r * 16 + fr] = p1[g][q];
-        }
-        if (ABS) {
-            double v = a1[g];
-            v += __shfl_xor(v, 16);
-            v += __shfl_xor(v, 32);
-            if (fk == 0 && r0 + fr < NB) A1[r0 + fr] = v;
+        for (int tt = 0; tt < 2; tt++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int r = r0 + 2 * MF<double>::row(q, fk) + tt;
+                if (r < NB) P1[(int64_t)r * 16 + fr] = p1[g][tt][q];
+            }
+            if (ABS) {
+                double v = a1[g][tt];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (fk == 0 && r0 + 2 * fr + tt < NB) A1[r0 + 2 * fr + tt] = v;
+            }
         }
     }
 }
@@ -463,7 +477,9 @@ int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS
                 const double *dB, int ldb, double *dR, int ldr, double *rowabs)
 {
     const int MT = (N + NB - 1) / NB;
-    if (NB >= 64 && NB <= 512) {     // below 64 the partials outweigh the second read
+    // below NB = 64 the partials outweigh the second read; the row-pair
+    // loads need an even NB (16-byte aligned pairs)
+    if (NB >= 64 && NB <= 512 && NB % 2 == 0) {
         const int64_t ntile = (int64_t)MT * (MT + 1) / 2;
         double *P = (double *)g_part.get((size_t)ntile * SY_PART * NB * sizeof(double));
         if (!P) {

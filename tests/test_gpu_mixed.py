@@ -38,7 +38,8 @@ def nan_unreferenced(po, T, N, NB):
 
 
 @pytest.mark.parametrize("N,NB,nrhs", [(378, 93, 16), (1000, 256, 5), (700, 64, 1), (1100, 512, 17),
-                                       (1024, 512, 16), (300, 32, 16), (1500, 640, 3)])
+                                       (1024, 512, 16), (300, 32, 16), (1500, 640, 3),
+                                       (777, 96, 16), (1000, 250, 7)])
 def test_dsymm_resid(dev, po, N, NB, nrhs):
     rng = np.random.default_rng(N)
     A = rng.standard_normal((N, N))
