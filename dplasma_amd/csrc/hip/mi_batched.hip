@@ -173,7 +173,12 @@ __device__ __forceinline__ int diag_block(T *__restrict__ A, int n, int j0, int 
             const T m = (!lrow || r > c) ? sc : T(0);
             v[c] = lrow ? (r == c ? d : (r > c ? sc : v[c])) : sc;
 #pragma unroll
-            for (int c2 = c + 1; c2 < BW; c2++) v[c2] -= m * readlane_t(m, c2);
+            for (int c2 = c + 1; c2 < BW; c2++) {
+                v[c2] -= m * readlane_t(m, c2);
+                // at most four broadcasts in flight: keeps the scalar
+                // registers they land in from piling up (fewer spills)
+                if ((c2 & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
         }
         if (r == 0) sh.fail = (fail > 0 && fail <= jb) ? fail : 0;   // wave-uniform
         if (lrow) {
