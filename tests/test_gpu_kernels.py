@@ -272,7 +272,9 @@ def test_spotrf_info(dev, po):
 
 
 @pytest.mark.parametrize("N,NB,nrhs,trans", [(700, 128, 1, 111), (700, 128, 1, 112), (1000, 320, 16, 111),
-                                             (1000, 320, 16, 112), (378, 93, 5, 112)])
+                                             (1000, 320, 16, 112), (378, 93, 5, 112),
+                                             (3000, 256, 17, 111), (3000, 256, 17, 112),
+                                             (1000, 64, 16, 112), (1000, 96, 3, 111), (1000, 96, 3, 112)])
 def test_strsm_LL_kernel(dev, po, N, NB, nrhs, trans):
     """FP32 tile solve against the FP64 oracle on the same FP32 data."""
     L, T = _tri_tiles(po, N, NB, N + nrhs)
