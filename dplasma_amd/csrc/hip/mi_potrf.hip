@@ -705,19 +705,25 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
         }
     };
 
+    // MFMA row tiles 2p and 2p+1 of a wave interleave their rows: row i of
+    // tile 2p+t is row 32p + 2i + t of the wave's rows, so that a lane holds
+    // two consecutive rows of one column and C moves as element pairs
     // full off-diagonal blocks start from C (loaded with the first operands)
     // and end with a plain store; the others accumulate -A B^T from zero
     const int fr0 = lane & 15, fk0 = lane >> 4;
     const bool pre = full && !((m == n) && (bm == bn));
     if (pre) {
 #pragma unroll
-        for (int i = 0; i < TI; i++)
+        for (int p = 0; p < TI / 2; p++)
 #pragma unroll
             for (int j = 0; j < TI; j++) {
-                const int row = r_base + wm * HB + i * 16 + fr0;
+                const int row = r_base + wm * HB + 32 * p + 2 * fr0;
 #pragma unroll
-                for (int q = 0; q < 4; q++)
-                    acc[i][j][q] = Cp[(int64_t)(c_base + wn * HB + j * 16 + MF<T>::row(q, fk0)) * NB + row];
+                for (int q = 0; q < 4; q++) {
+                    const V2 c = *(const V2 *)&Cp[(int64_t)(c_base + wn * HB + j * 16 + MF<T>::row(q, fk0)) * NB + row];
+                    acc[2 * p][j][q] = c.x;
+                    acc[2 * p + 1][j][q] = c.y;
+                }
             }
     }
     const int nstage = (NB + BK - 1) / BK;
@@ -732,7 +738,11 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
         for (int kq = 0; kq < BK / 4; kq++) {
             T a[TI], b[TI];
 #pragma unroll
-            for (int i = 0; i < TI; i++) a[i] = As[buf][kq * 4 + fk][wm * HB + i * 16 + fr];
+            for (int p = 0; p < TI / 2; p++) {
+                const V2 v = *(const V2 *)&As[buf][kq * 4 + fk][wm * HB + 32 * p + 2 * fr];
+                a[2 * p] = v.x;
+                a[2 * p + 1] = v.y;
+            }
 #pragma unroll
             for (int j = 0; j < TI; j++) b[j] = Bs[buf][kq * 4 + fk][wn * HB + j * 16 + fr];
 #pragma unroll
@@ -746,18 +756,26 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
     }
 
     // epilogue: the MFMA was fed (B, A), so element q of lane l holds row
-    // l % 16 and column row(q, l / 16) of each 16x16 tile: a store walks rows
+    // 2 (l % 16) + t (tile 2p + t) and column row(q, l / 16) of each 16x16
+    // tile: a store walks row pairs
     const bool diag = (m == n) && (bm == bn);
 #pragma unroll
-    for (int i = 0; i < TI; i++) {
-        const int rr = r_base + wm * HB + i * 16 + fr;
+    for (int p = 0; p < TI / 2; p++) {
+        const int rr = r_base + wm * HB + 32 * p + 2 * fr;
 #pragma unroll
         for (int j = 0; j < TI; j++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int col = c_base + wn * HB + j * 16 + MF<T>::row(q, fk);
-                if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
-                else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
+                if (pre) {
+                    *(V2 *)&Cp[(int64_t)col * NB + rr] = V2{acc[2 * p][j][q], acc[2 * p + 1][j][q]};
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 2; t++) {
+                        const int r = rr + t;
+                        if (r < rows_m && col < cols_n && (!diag || r >= col)) Cp[(int64_t)col * NB + r] += acc[2 * p + t][j][q];
+                    }
+                }
             }
     }
 }
