@@ -114,9 +114,9 @@ int mi_dtrsm_LL(mi_ctx_t *ctx, int trans, int diag, double alpha, int N, int NB,
 /* B <- (L L^T)^{-1} B: src/zpotrs_wrapper.c:91-94.  Blocking. */
 int mi_dpotrs_tiles(mi_ctx_t *ctx, int N, int NB, const double *dT, int NRHS, double *dB, int ldb);
 /* Solve engine: 1 (default) = one wavefront launch per 16 right-hand sides
- * (workgroups of 128 rows chained by device-scope flags) whenever
- * NB % 32 == 0; 0 = one diagonal-solve and one update launch per tile.
- * Process-wide. */
+ * (workgroups of 128 rows, each handing its rows of X to the next ones
+ * through uncached device memory) whenever NB % 32 == 0; 0 = one
+ * diagonal-solve and one update launch per tile.  Process-wide. */
 int mi_set_solve_engine(int which);
 /* FP32 tiles and right-hand sides: the "s" precision of the two calls above. */
 int mi_strsm_LL(mi_ctx_t *ctx, int trans, int diag, float alpha, int N, int NB,
