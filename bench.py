@@ -20,7 +20,9 @@ FP32, plus 64 GPs of 1024 points), the roofline of the dominant kernel (the
 trailing update, timed with HIP events on its own stream in a separate
 profiled step) and the CPU baseline:
 the oracle's tile-DAG executor (a restatement of src/zpotrf_L.jdf with
-OpenBLAS tile kernels) on the host cores, on a bounded sample.
+OpenBLAS tile kernels) on the host cores, on a bounded sample.  When that
+sample is the full N = 32768 workload, its factor is compared element by
+element with the GPU factor of the same matrix (vs_cpu_oracle).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -46,7 +48,9 @@ def flops_dpotrf(n):
 
 def cpu_baseline(seconds_hint=True):
     """Oracle tile-DAG dpotrf on the host cores (test infrastructure, timed
-    only as the reported CPU baseline)."""
+    only as the reported CPU baseline).  Returns the baseline line and, when
+    the full N=32768 workload ran, the oracle's factor (packed lower tiles)
+    for the comparison with the GPU factor."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     blas = po.load_openblas()
@@ -60,16 +64,16 @@ def cpu_baseline(seconds_hint=True):
         T = po.dplgsy_symtiles(N, NB, float(N), 3872)
         t0 = time.time()
         info = po.dpotrf_tiles(T, N, NB, nthreads=cores, use_blas=blas is not None)
-        return time.time() - t0, info
+        return time.time() - t0, info, T
 
     # a 1/8-flop sample first; the full N=32768 workload when that took < 4 s
     N = 16384
-    dt, info = run(N)
+    dt, info, T = run(N)
     if dt < 4.0:
         N = 32768
-        dt, info = run(N)
+        dt, info, T = run(N)
     what = "the full N=32768 workload" if N == 32768 else "1/8 of the N=32768 flops"
-    return {
+    return (T if N == 32768 and info == 0 else None), {
         "value": round(flops_dpotrf(N) / dt / 1e12, 4),
         "unit": "TFLOP/s",
         "cores": cores,
@@ -78,6 +82,25 @@ def cpu_baseline(seconds_hint=True):
                    f"tile-DAG of zpotrf_L.jdf with {os.path.basename(blas) if blas else 'plain C'} "
                    f"tile kernels, {dt:.2f} s, info={info}"),
     }
+
+
+def compare_lower_tiles(G, C, N, NB):
+    """Largest difference between two packed lower-tile factors over the
+    referenced elements (off-diagonal tiles whole, diagonal tiles on and
+    below the diagonal; N a multiple of NB)."""
+    import numpy as np
+    MT = N // NB
+    g = G[:MT * (MT + 1) // 2 * NB * NB].reshape(-1, NB, NB)
+    c = C[:MT * (MT + 1) // 2 * NB * NB].reshape(-1, NB, NB)
+    diag = np.array([n * MT - n * (n - 1) // 2 for n in range(MT)])
+    off = np.setdiff1d(np.arange(g.shape[0]), diag)
+    d_off = max((np.abs(g[i] - c[i]).max() for i in np.array_split(off, 64) if len(i)), default=0.0)
+    low = np.triu(np.ones((NB, NB), dtype=bool))      # [col][row]: row >= col
+    d_diag = max(np.abs(g[i] - c[i])[low].max() for i in diag)
+    scale = max(np.abs(c[off[:64]]).max(), max(np.abs(c[i])[low].max() for i in diag))
+    d = max(d_off, d_diag)
+    return {"max_abs_diff": float(d), "max_abs_L": float(scale), "rel": float(d / scale),
+            "what": "GPU dpotrf factor vs the CPU oracle's factor of the same matrix (N=32768, NB=512)"}
 
 
 def mixed_leg(dp, N, NB, nrhs=16, kappas=(1e2, 1e5, 1e8)):
@@ -494,6 +517,12 @@ def main():
     ctx.sync_host(X)
     chk_s = ctx.check_daxmb(0, dp.LOWER, A0, B, X)
 
+    # the GPU factor on the host, for the comparison with the CPU oracle's
+    L_gpu = None
+    if not args.no_cpu and N % NB == 0:
+        assert ctx.sync_host(A) == 0
+        L_gpu = A.array.copy()
+
     # adjoint of the same factorization: Lbar = a dplgsy-pattern matrix (seed 11), in HBM
     Gm = A0                                      # reuse: A0 is no longer needed
     assert ctx.dplgsy(1.0, dp.LOWER, Gm, 11) == 0
@@ -547,7 +576,9 @@ def main():
     if not args.no_gp:
         line["gp_nll"] = gp_leg(dp)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
+        T_cpu, line["cpu_baseline"] = cpu_baseline()
+        if T_cpu is not None and L_gpu is not None and N == 32768:
+            line["vs_cpu_oracle"] = compare_lower_tiles(L_gpu, T_cpu, N, NB)
     print(json.dumps(line))
 
 
