@@ -300,6 +300,18 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
 
     for (int j0 = 0; j0 < NB; j0 += 32) {
         const int jw = min(32, NB - j0);
+        // this block's columns of C (written only by this step) and W_jj do
+        // not depend on the solve so far: in flight during the product
+        T cb[2][4], wv[4];
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = wave * 16 + MF<T>::row(q, fk), c = ct * 16 + fr;
+                cb[ct][q] = (rr < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + rr] : T(0);
+            }
+#pragma unroll
+        for (int u = 0; u < 4; u++) wv[u] = Winv[(j0 >> 5) * 1024 + tid + 256 * u];
         V4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
         if (j0 > 0) mfma_panel_2(C, NB, wave * 16, rv, L + j0, NB, 0, jw, j0, acc);
 #pragma unroll
@@ -307,9 +319,10 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int rr = wave * 16 + MF<T>::row(q, fk), c = ct * 16 + fr;
-                P[c][rr] = (rr < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + rr] - acc[ct][q] : T(0);
+                P[c][rr] = (rr < rv && c < jw) ? cb[ct][q] - acc[ct][q] : T(0);
             }
-        for (int idx = tid; idx < 1024; idx += 256) Ws[idx] = Winv[(j0 >> 5) * 1024 + idx];
+#pragma unroll
+        for (int u = 0; u < 4; u++) Ws[tid + 256 * u] = wv[u];
         __syncthreads();
 #pragma unroll
         for (int jj = 0; jj < 8; jj++) {
@@ -349,6 +362,16 @@ __global__ __launch_bounds__(256) void k_trsm_tile_ks(T *__restrict__ Tt, int N,
 
     for (int j0 = 0; j0 < NB; j0 += 32) {
         const int jw = min(32, NB - j0);
+        // this block's columns of C (written only by this step) and W_jj do
+        // not depend on the solve so far: in flight during the product
+        T cb[2], wv[4];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int idx = tid + 256 * u, r = idx % TSK_ROWS, c = idx / TSK_ROWS;
+            cb[u] = (r < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + r] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) wv[u] = Winv[(j0 >> 5) * 1024 + tid + 256 * u];
         // this wave's K slice of the j0 solved columns
         const int np = j0 / 32;
         const int p0 = (np * wave) / 4, p1 = (np * (wave + 1)) / 4;
@@ -360,12 +383,14 @@ __global__ __launch_bounds__(256) void k_trsm_tile_ks(T *__restrict__ Tt, int N,
         for (int ct = 0; ct < 2; ct++)
 #pragma unroll
             for (int q = 0; q < 4; q++) red[wave][ct * 16 + fr][MF<T>::row(q, fk)] = acc[ct][q];
-        for (int idx = tid; idx < 1024; idx += 256) Ws[idx] = Winv[(j0 >> 5) * 1024 + idx];
+#pragma unroll
+        for (int u = 0; u < 4; u++) Ws[tid + 256 * u] = wv[u];
         __syncthreads();
-        for (int idx = tid; idx < 32 * TSK_ROWS; idx += 256) {
-            const int r = idx % TSK_ROWS, c = idx / TSK_ROWS;
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int idx = tid + 256 * u, r = idx % TSK_ROWS, c = idx / TSK_ROWS;
             const T sum = red[0][c][r] + red[1][c][r] + red[2][c][r] + red[3][c][r];
-            P[c][r] = (r < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + r] - sum : T(0);
+            P[c][r] = (r < rv && c < jw) ? cb[u] - sum : T(0);
         }
         __syncthreads();
         for (int idx = tid; idx < 32 * TSK_ROWS; idx += 256) {
