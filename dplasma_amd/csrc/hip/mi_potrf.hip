@@ -324,12 +324,26 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
 #pragma unroll
         for (int u = 0; u < 4; u++) Ws[tid + 256 * u] = wv[u];
         __syncthreads();
+        // X = P W_jj^T on MFMA (W_jj is zero above its diagonal), fed (W, P)
+        // so that element q of lane l holds row 16 w + l % 16 and column
+        // 16 ct + row(q, l / 16): the stores walk rows
+        {
+            V4 xo[2] = {{T(0), T(0), T(0), T(0)}, {T(0), T(0), T(0), T(0)}};
 #pragma unroll
-        for (int jj = 0; jj < 8; jj++) {
-            const int c = cg * 8 + jj;
-            T s = T(0);
-            for (int p = 0; p <= c; p++) s += P[p][r] * Ws[p * 32 + c];
-            if (r < rv && c < jw) C[(int64_t)(j0 + c) * NB + r] = s;
+            for (int s4 = 0; s4 < 8; s4++) {
+                const int kk = 4 * s4 + fk;
+                const T a = P[kk][wave * 16 + fr];
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) xo[ct] = MF<T>::mfma(Ws[kk * 32 + ct * 16 + fr], a, xo[ct]);
+            }
+            const int rr = wave * 16 + fr;
+#pragma unroll
+            for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int c = ct * 16 + MF<T>::row(q, fk);
+                    if (rr < rv && c < jw) C[(int64_t)(j0 + c) * NB + rr] = xo[ct][q];
+                }
         }
         __syncthreads();
     }
@@ -393,11 +407,20 @@ __global__ __launch_bounds__(256) void k_trsm_tile_ks(T *__restrict__ Tt, int N,
             P[c][r] = (r < rv && c < jw) ? cb[u] - sum : T(0);
         }
         __syncthreads();
-        for (int idx = tid; idx < 32 * TSK_ROWS; idx += 256) {
-            const int r = idx % TSK_ROWS, c = idx / TSK_ROWS;
-            T x = T(0);
-            for (int p = 0; p <= c; p++) x += P[p][r] * Ws[p * 32 + c];
-            if (r < rv && c < jw) C[(int64_t)(j0 + c) * NB + r] = x;
+        // X = P W_jj^T on MFMA (W_jj is zero above its diagonal): waves 0
+        // and 1 take 16 columns each, fed (W, P) so that the stores walk rows
+        if (wave < 2) {
+            V4 xo = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int s4 = 0; s4 < 8; s4++) {
+                const int kk = 4 * s4 + fk;
+                xo = MF<T>::mfma(Ws[kk * 32 + wave * 16 + fr], P[kk][fr], xo);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int c = wave * 16 + MF<T>::row(q, fk);
+                if (fr < rv && c < jw) C[(int64_t)(j0 + c) * NB + fr] = xo[q];
+            }
         }
         __syncthreads();
     }
