@@ -353,10 +353,11 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
 // 16-row strip; the left-looking product X[:, 0:j0] L[j0:j0+32, 0:j0]^T of
 // each 32-column block is split over the 4 waves by K (contiguous 32-column
 // slices, partial sums met in LDS), so the per-block chain is a quarter of
-// the 64-row kernel's; then P = C - sum, X = P W_jj^T by all threads.
+// the 64-row kernel's; then P = C - sum and X = P W_jj^T on MFMA.  Bounded
+// to two workgroups per CU (256 registers a lane) without spills.
 constexpr int TSK_ROWS = 16;
 template <class T>
-__global__ __launch_bounds__(256) void k_trsm_tile_ks(T *__restrict__ Tt, int N, int NB, int MT, int k,
+__global__ __launch_bounds__(256, 2) void k_trsm_tile_ks(T *__restrict__ Tt, int N, int NB, int MT, int k,
                                                      const T *__restrict__ Winv)
 {
     __shared__ T red[4][32][TSK_ROWS + 1];       // partial products [wave][col][row]
@@ -1179,8 +1180,10 @@ int launch_trsm(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, cons
     double fl = 0.0;
     for (int m = k + 1; m < MT; m++) fl += (double)tile_rows(N, NB, MT, m) * NB * NB;
     if (!valu_panel<T>()) {
+        // 64-row strips while there are many tiles; below ~50 tiles (FP64)
+        // or ~40 (FP32) of NB = 512 the 16-row K-split kernel is faster alone
         const int strips = (NB + TS_ROWS - 1) / TS_ROWS;
-        if (ntile * strips >= 256) {
+        if (ntile * strips >= (sizeof(T) == 8 ? 400 : 320)) {
             MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
                       k_trsm_tile<T><<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
         } else {
