@@ -114,6 +114,11 @@ def libs():
     _sig(M, "mi_dpotrf_tiles", i, [vp, i, i, vp, pi])
     _sig(M, "mi_dtrsm_LL", i, [vp, i, i, d, i, i, vp, i, vp, i])
     _sig(M, "mi_dpotrs_tiles", i, [vp, i, i, vp, i, vp, i])
+    _sig(M, "mi_dpotrf_lapack", i, [vp, i, i, vp, i, i, pi])
+    _sig(M, "mi_dpotrs_lapack", i, [vp, i, i, i, vp, i, vp, i, i])
+    _sig(M, "mi_dposv_lapack", i, [vp, i, i, i, vp, i, vp, i, i, pi])
+    _sig(M, "mi_dsym_lapack_to_tiles", i, [vp, i, i, vp, i, i, vp])
+    _sig(M, "mi_dsym_tiles_to_lapack", i, [vp, i, i, i, vp, vp, i])
     _sig(M, "mi_spotrf_tiles", i, [vp, i, i, vp, pi])
     _sig(M, "mi_spotrf_step_potrf", i, [vp, i, i, vp, i, pi])
     _sig(M, "mi_dsposv_tiles", i, [vp, i, i, vp, i, vp, i, vp, i, i, pi, pi])

@@ -123,6 +123,31 @@ int mi_strsm_LL(mi_ctx_t *ctx, int trans, int diag, float alpha, int N, int NB,
                 const float *dT, int NRHS, float *dB, int ldb);
 int mi_spotrs_tiles(mi_ctx_t *ctx, int N, int NB, const float *dT, int NRHS, float *dB, int ldb);
 
+/* LAPACK-layout entries: A is a column-major N x N array in device memory
+ * (lda >= max(1, N)) of which only the uplo triangle (121 Upper, 122 Lower)
+ * is read and written; B is column-major (ldb >= max(1, N)).  They replace
+ * the reference's route to LAPACK-layout memory -- a block-cyclic descriptor
+ * with PARSEC_MATRIX_LAPACK storage over the caller's array, as the
+ * ScaLAPACK-style pdpotrf_w builds it
+ * (src/scalapack_wrappers/dplasma_wrapper_pdpotrf.c:139-232), staged tile by
+ * tile with 2-D pitched copies (src/cuda/lapack_cuda_stage_in.c:18-162) --
+ * by one device staging pass each way into a packed lower tile image of
+ * NB x NB tiles (allocated per call) and the tile path above.
+ * mi_dpotrf_lapack: A = L L^T (Lower) or U^T U (Upper) in place, *info as
+ * mi_dpotrf_tiles (the triangle is written back also when *info > 0).
+ * mi_dpotrs_lapack: B <- A^{-1} B from that factor.  mi_dposv_lapack: both,
+ * the solve only when *info == 0.  Blocking. */
+int mi_dpotrf_lapack(mi_ctx_t *ctx, int uplo, int N, double *dA, int lda, int NB, int *info);
+int mi_dpotrs_lapack(mi_ctx_t *ctx, int uplo, int N, int NRHS, const double *dA, int lda, double *dB,
+                     int ldb, int NB);
+int mi_dposv_lapack(mi_ctx_t *ctx, int uplo, int N, int NRHS, double *dA, int lda, double *dB, int ldb,
+                    int NB, int *info);
+/* The staging passes alone: the uplo triangle of column-major A to the
+ * packed lower tile image dT (entries outside the triangle and beyond N
+ * zero), and the lower triangle of dT back to A's uplo triangle. */
+int mi_dsym_lapack_to_tiles(mi_ctx_t *ctx, int uplo, int N, const double *dA, int lda, int NB, double *dT);
+int mi_dsym_tiles_to_lapack(mi_ctx_t *ctx, int uplo, int N, int NB, const double *dT, double *dA, int lda);
+
 /* Single task-class launches (one step k of the factorization), for tests:
  *   POTRF(k)            src/zpotrf_L.jdf:91-208
  *   TRSM(k+1..MT-1, k)  src/zpotrf_L.jdf:214-343

@@ -78,3 +78,25 @@ def test_no_gpu_is_reported_not_hidden(dp):
         pytest.skip("a GPU is visible")
     with pytest.raises(RuntimeError):
         dp.Context(0)
+
+
+def test_lapack_entries_reject_bad_arguments_before_touching_the_device(dp):
+    """The LAPACK-layout entries validate uplo, N, lda, NB (and NRHS, ldb)
+    first, as LAPACK's xerbla checks do, with no context or device needed."""
+    M, _ = dp.libs()
+    info = ctypes.c_int(5)
+    assert M.mi_dpotrf_lapack(None, 0, 10, None, 10, 32, ctypes.byref(info)) == -2
+    assert info.value == 0
+    assert M.mi_dpotrf_lapack(None, 122, -1, None, 10, 32, ctypes.byref(info)) == -3
+    assert M.mi_dpotrf_lapack(None, 121, 10, None, 9, 32, ctypes.byref(info)) == -5
+    assert M.mi_dpotrf_lapack(None, 122, 10, None, 10, 0, ctypes.byref(info)) == -6
+    assert M.mi_dpotrs_lapack(None, 122, 10, -1, None, 10, None, 10, 32) == -4
+    assert M.mi_dpotrs_lapack(None, 122, 10, 1, None, 10, None, 9, 32) == -8
+    assert M.mi_dposv_lapack(None, 123, 10, 1, None, 10, None, 10, 32, ctypes.byref(info)) == -2
+    assert M.mi_dposv_lapack(None, 122, 10, 1, None, 10, None, 3, 32, ctypes.byref(info)) == -8
+    assert M.mi_dsym_lapack_to_tiles(None, 122, 10, None, 4, 32, None) == -5
+    assert "lda = 4" in dp.last_error()
+    assert M.mi_dsym_tiles_to_lapack(None, 121, 10, 32, None, None, 4) == -5
+    # N = 0 is a no-op that succeeds without a device
+    assert M.mi_dpotrf_lapack(None, 122, 0, None, 1, 32, ctypes.byref(info)) == 0 and info.value == 0
+    assert M.mi_dpotrs_lapack(None, 121, 0, 3, None, 1, None, 1, 32) == 0
