@@ -296,7 +296,6 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
     const int rv = min(TS_ROWS, rows_m - r0);
     T *C = Tt + tile_index(MT, m, k) * (int64_t)NB * NB + r0;
     const T *L = Tt + tile_index(MT, k, k) * (int64_t)NB * NB;
-    const int r = tid & 63, cg = tid >> 6;
 
     for (int j0 = 0; j0 < NB; j0 += 32) {
         const int jw = min(32, NB - j0);
