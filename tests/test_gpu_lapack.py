@@ -10,6 +10,9 @@ Cholesky (numpy) to a relative 1e-12; the unreferenced triangle and the rows
 beyond N are not touched (they hold NaN, so a read would also show in the
 factor); the solve's backward error is below 1e-14 and equals the tile
 solve bit for bit; info on an indefinite matrix is the failing column + 1.
+The host route (dplasma_dpotrf on a descriptor with LAPACK storage) is
+checked against LAPACK's Cholesky with NaN in the other triangle, which the
+host staging carries into the unread halves of the diagonal tiles.
 """
 import ctypes
 
@@ -145,3 +148,27 @@ def test_lapack_staging_round_trip(dev, po, uplo):
     assert np.array_equal(G[mask], colmajor(A, lda, uplo)[mask])
     for p in (pA, pT, pA2):
         dev.free(p)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("N,NB", [(378, 93), (1000, 256)])
+def test_dplasma_dpotrf_on_lapack_storage_descriptor(ctx, dp, po, N, NB, uplo):
+    """dplasma_dpotrf on a block-cyclic descriptor with PARSEC_MATRIX_LAPACK
+    storage over a column-major host array: the call pdpotrf_w makes
+    (src/scalapack_wrappers/dplasma_wrapper_pdpotrf.c:180-229).  The factor
+    matches LAPACK's to a relative 1e-12; the other triangle (NaN) is neither
+    read nor written."""
+    A0 = po.dplgsy_full(N, float(N), 3872)
+    u = dp.LOWER if uplo == "L" else dp.UPPER
+    keep = np.tril(np.ones((N, N), bool)) if uplo == "L" else np.triu(np.ones((N, N), bool))
+    H = np.where(keep, A0, np.nan)
+    A = dp.GeMatrix(ctx, N, N, NB, NB, storage=dp.MATRIX_LAPACK)
+    A.from_numpy(H)
+    assert ctx.dpotrf(u, A) == 0
+    G = A.to_numpy()
+    L = np.linalg.cholesky(A0)
+    F = np.where(keep, G, 0.0)
+    F = F if uplo == "L" else F.T
+    assert np.abs(F - L).max() <= 1e-12 * np.abs(L).max()
+    assert np.isnan(G[~keep]).all()
+    A.free()
