@@ -171,4 +171,13 @@ def test_dplasma_dpotrf_on_lapack_storage_descriptor(ctx, dp, po, N, NB, uplo):
     F = F if uplo == "L" else F.T
     assert np.abs(F - L).max() <= 1e-12 * np.abs(L).max()
     assert np.isnan(G[~keep]).all()
+    # dplasma_dpotrs on the factor, right-hand sides in LAPACK storage too
+    B0 = po.dplrnt_full(N, 3, 2354)
+    B = dp.GeMatrix(ctx, N, 3, NB, NB, storage=dp.MATRIX_LAPACK)
+    B.from_numpy(B0)
+    assert ctx.dpotrs(u, A, B) == 0
+    X = B.to_numpy()
+    r = np.abs(B0 - A0 @ X).max() / (np.abs(A0).sum(1).max() * np.abs(X).max() + np.abs(B0).max())
+    assert r < 1e-14, r
     A.free()
+    B.free()
