@@ -22,7 +22,9 @@ profiled step) and the CPU baseline:
 the oracle's tile-DAG executor (a restatement of src/zpotrf_L.jdf with
 OpenBLAS tile kernels) on the host cores, on a bounded sample.  When that
 sample is the full N = 32768 workload, its factor is compared element by
-element with the GPU factor of the same matrix (vs_cpu_oracle).
+element with the GPU factor of the same matrix (vs_cpu_oracle).  The
+reference's own CPU-runnable case (N = 4096, NB = 256) is timed the same two
+ways, with its two factors compared (dpotrf_N4096_NB256).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -82,6 +84,52 @@ def cpu_baseline(seconds_hint=True):
                    f"tile-DAG of zpotrf_L.jdf with {os.path.basename(blas) if blas else 'plain C'} "
                    f"tile kernels, {dt:.2f} s, info={info}"),
     }
+
+
+def small_case_leg(dp, N=4096, NB=256, runs=5):
+    """The reference's own CPU-runnable dpotrf case (Lower, N=4096, NB=256,
+    dplgsy seed 3872), timed on the GPU (event-timed, resident tiles) and
+    with the oracle's tile DAG on the host cores, and the two factors
+    compared element by element."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    blas = po.load_openblas()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    M, _ = dp.libs()
+    MT = (N + NB - 1) // NB
+    nt = MT * (MT + 1) // 2 * NB * NB
+    gpu_ms = []
+    with dp.Device(0) as dev:
+        dA = dev.alloc(nt * 8)
+        info = ctypes.c_int()
+        ms = ctypes.c_double()
+        for _ in range(runs + 1):                  # the first run warms up
+            dev.check(M.mi_dplgsy_tiles(dev.h, float(N), N, NB, dA, 3872), "dplgsy")
+            dev.check(M.mi_dpotrf_tiles(dev.h, N, NB, dA, ctypes.byref(info)), "dpotrf")
+            M.mi_last_span_ms(dev.h, ctypes.byref(ms))
+            gpu_ms.append(ms.value)
+        G = dev.download(dA, nt)
+        dev.free(dA)
+    cpu_s = []
+    for _ in range(3):
+        T = po.dplgsy_symtiles(N, NB, float(N), 3872)
+        t0 = time.time()
+        cinfo = po.dpotrf_tiles(T, N, NB, nthreads=cores, use_blas=blas is not None)
+        cpu_s.append(time.time() - t0)
+    g_ms = float(np.median(gpu_ms[1:]))
+    c_ms = float(np.median(cpu_s)) * 1e3
+    cmp = compare_lower_tiles(G, T, N, NB)
+    cmp["what"] = f"GPU dpotrf factor vs the CPU oracle's factor of the same matrix (N={N}, NB={NB})"
+    return {"N": N, "NB": NB, "gpu_ms": round(g_ms, 3), "gpu_tflops": round(flops_dpotrf(N) / g_ms / 1e9, 3),
+            "gpu_info": info.value, "cpu_ms": round(c_ms, 2), "cpu_tflops": round(flops_dpotrf(N) / c_ms / 1e9, 4),
+            "cpu_cores": cores, "cpu_info": cinfo,
+            "cpu_kernels": os.path.basename(blas) if blas else "plain C", "factor_vs_cpu_oracle": cmp}
 
 
 def compare_lower_tiles(G, C, N, NB):
@@ -579,6 +627,7 @@ def main():
         T_cpu, line["cpu_baseline"] = cpu_baseline()
         if T_cpu is not None and L_gpu is not None and N == 32768:
             line["vs_cpu_oracle"] = compare_lower_tiles(L_gpu, T_cpu, N, NB)
+        line["dpotrf_N4096_NB256"] = small_case_leg(dp)
     print(json.dumps(line))
 
 
