@@ -68,6 +68,10 @@ class Timing(ctypes.Structure):
                 ("stage_out_ms", ctypes.c_double)]
 
 
+class CheckNorms(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("Anorm", "Rnorm", "Bnorm", "Xnorm", "Ainvnorm", "result")]
+
+
 def _sig(lib, name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -157,6 +161,7 @@ def libs():
     _sig(M, "mi_set_panel_engine", i, [i])
     _sig(M, "mi_set_spotrf_update", i, [i])
     _sig(M, "mi_set_solve_engine", i, [i])
+    _sig(M, "mi_set_solve_wait_limit", i, [d])
     _sig(M, "mi_dcheck_potrf", i, [vp, i, i, vp, vp, pd, pd])
     _sig(M, "mi_dcheck_axmb", i, [vp, i, i, vp, i, vp, i, vp, i, pd, pd, pd, pd])
     _sig(M, "mi_prof_enable", i, [vp, i])
@@ -206,6 +211,7 @@ def libs():
     _sig(D, "dplasma_data_sync_host", i, [vp, vp])
     _sig(D, "dplasma_data_release", i, [vp, vp])
     _sig(D, "dplasma_mi_last_timing", i, [vp, ctypes.POINTER(Timing)])
+    _sig(D, "dplasma_mi_last_check", i, [vp, ctypes.POINTER(CheckNorms)])
     _sig(D, "dplasma_mi_context", vp, [vp])
     return _mi, _dpl
 
@@ -406,6 +412,12 @@ class Context:
         t = Timing()
         self.D.dplasma_mi_last_timing(self.ptr, ctypes.byref(t))
         return {"total_ms": t.total_ms, "stage_in_ms": t.stage_in_ms, "stage_out_ms": t.stage_out_ms}
+
+    def last_check(self):
+        """Norms and ratio behind the last check_* call on this context."""
+        c = CheckNorms()
+        self.D.dplasma_mi_last_check(self.ptr, ctypes.byref(c))
+        return {n: getattr(c, n) for n, _ in CheckNorms._fields_}
 
     # -- kernel profiler of the underlying kernel context
     def prof_enable(self, on=True):

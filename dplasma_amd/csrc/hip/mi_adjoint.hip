@@ -382,21 +382,6 @@ template <> struct AN<float> {
     static constexpr const char *aux = "spotrf_adjoint_aux";
 };
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    void *get(size_t need)
-    {
-        if (bytes >= need) return p;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        if (hipMalloc(&p, need) != hipSuccess) { p = nullptr; return nullptr; }
-        bytes = need;
-        return p;
-    }
-};
-DevBuf g_aw, g_apart, g_ajobs, g_asegs;
 
 constexpr int64_t PART_CAP = (int64_t)96 << 20;   // bytes of split-K partials per launch
 
@@ -495,8 +480,8 @@ int adjoint_tiles(mi_ctx_t *c, int N, int NB, const T *dL, T *dG)
     // workspace: W (MT tiles), D (MT tiles), Pb (MT tiles; also the scratch
     // of the inversion), Q, P, X (one tile each)
     const size_t ws_elems = (size_t)(3 * MT + 3) * T2;
-    T *ws = (T *)g_aw.get(ws_elems * sizeof(T));
-    if (!ws) { set_error("out of device memory for the adjoint workspace"); return -3; }
+    T *ws = (T *)ws_get(c, WS_ADJ_W, ws_elems * sizeof(T));
+    if (!ws) return -3;
     T *W = ws, *D = W + MT * T2, *Pb = D + MT * T2, *Q = Pb + MT * T2, *P = Q + T2, *X = P + T2;
     auto tile = [&](const T *base, int m, int n) { return const_cast<T *>(base) + tile_index(MT, m, n) * T2; };
 
@@ -609,21 +594,21 @@ int adjoint_tiles(mi_ctx_t *c, int N, int NB, const T *dL, T *dG)
     // upload the job and segment tables (one synchronous copy each)
     GSeg<T> *dsegs = nullptr;
     if (!pl.segs.empty()) {
-        dsegs = (GSeg<T> *)g_asegs.get(pl.segs.size() * sizeof(GSeg<T>));
-        if (!dsegs) { set_error("out of device memory for the adjoint plan"); return -3; }
+        dsegs = (GSeg<T> *)ws_get(c, WS_ADJ_SEGS, pl.segs.size() * sizeof(GSeg<T>));
+        if (!dsegs) return -3;
         MI_CHECK(hipMemcpy(dsegs, pl.segs.data(), pl.segs.size() * sizeof(GSeg<T>), hipMemcpyHostToDevice));
     }
     for (GJob<T> &J : pl.jobs)
         if (J.segs) J.segs = dsegs + ((uintptr_t)J.segs - 1);
-    GJob<T> *djobs = (GJob<T> *)g_ajobs.get(std::max<size_t>(1, pl.jobs.size()) * sizeof(GJob<T>));
-    if (!djobs) { set_error("out of device memory for the adjoint plan"); return -3; }
+    GJob<T> *djobs = (GJob<T> *)ws_get(c, WS_ADJ_JOBS, std::max<size_t>(1, pl.jobs.size()) * sizeof(GJob<T>));
+    if (!djobs) return -3;
     MI_CHECK(hipMemcpy(djobs, pl.jobs.data(), pl.jobs.size() * sizeof(GJob<T>), hipMemcpyHostToDevice));
     int64_t part_elems = 1;
     for (const Launch &l : pl.plan)
         if (l.kind == 0 && l.ks > 1)
             part_elems = std::max<int64_t>(part_elems, (int64_t)l.njobs * l.nsub * l.nsub * l.ks * l.bmx * l.bmx);
-    T *part = (T *)g_apart.get((size_t)part_elems * sizeof(T));
-    if (!part) { set_error("out of device memory for the adjoint partials"); return -3; }
+    T *part = (T *)ws_get(c, WS_ADJ_PART, (size_t)part_elems * sizeof(T));
+    if (!part) return -3;
 
     MI_CHECK(hipEventRecord(c->t0, s));
     for (const Launch &l : pl.plan) {
@@ -710,7 +695,6 @@ __global__ __launch_bounds__(256) void k_copy_lower_tiles(const double *__restri
     }
 }
 
-DevBuf g_pinv;
 
 }  // namespace
 
@@ -725,8 +709,8 @@ int mi_dpotri_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
     if (NB <= 0) { set_error("bad tile size"); return -2; }
     const int MT = (N + NB - 1) / NB;
     const int64_t tot = (int64_t)MT * (MT + 1) / 2 * NB * NB;
-    double *G = (double *)g_pinv.get((size_t)tot * sizeof(double));
-    if (!G) { set_error("out of device memory for the inverse"); return -3; }
+    double *G = (double *)ws_get(c, WS_POTRI, (size_t)tot * sizeof(double));
+    if (!G) return -3;
     int h_zero = 0;
     k_fill<int><<<1, 64, 0, c->s_main>>>(c->d_info, 1, N + 1);
     k_logdet_seed<<<(unsigned)std::min<int64_t>(65536, (tot + 255) / 256), 256, 0, c->s_main>>>(dT, N, NB, MT, G,

@@ -79,6 +79,30 @@ struct ProfAgg {
     double ms = 0.0, flops = 0.0, bytes = 0.0;
 };
 
+// Device workspaces owned by a context: grown on demand (ws_get), freed with
+// the context, so that calls on different contexts never share memory.
+enum WsId {
+    WS_POTRF_WINV = 0,  // diagonal-block inverses handed from POTRF(k) to TRSM(k)
+    WS_TRSM,            // solve: 32x32 and 128x128 block inverses, link matrices
+    WS_MX_L32,          // mixed solve: FP32 tiles
+    WS_MX_R32,          //   FP32 right-hand sides / corrections
+    WS_MX_R64,          //   FP64 residual, norms, row sums
+    WS_MX_W32,          //   FP32 solve workspace
+    WS_MX_PART,         //   per-tile partials of the symmetric residual
+    WS_MX_AUX,          //   FP64 vectors of the preconditioned iteration
+    WS_ADJ_W,           // adjoint: inverses and tile scratch
+    WS_ADJ_PART,        //   K-split partials
+    WS_ADJ_JOBS,        //   job table
+    WS_ADJ_SEGS,        //   segment table
+    WS_POTRI,           // inverse: the seed / result image
+    WS_DTD,             // task-insertion front-end: per-task diagonal-block inverses
+    WS_COUNT
+};
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t s_main = nullptr;   // trailing updates, solves, copies
@@ -88,6 +112,10 @@ struct Ctx {
     int *d_info = nullptr;          // device-side first-failure info
     double *d_scratch = nullptr;    // small reductions (norms)
     size_t scratch_bytes = 0;
+    DevBuf ws[WS_COUNT];
+    // hand-off memory of the wavefront solve (uncached; an uncached buffer
+    // that a kernel has read is never freed, see ws_uncached)
+    DevBuf wv_ticket, wv_stage;
     bool prof = false;
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> prof_free;
@@ -98,6 +126,14 @@ struct Ctx {
 
 void set_error(const char *fmt, ...);
 hipEvent_t next_event(Ctx *c);
+// Workspace `id` of the context with at least `bytes` bytes (nullptr and an
+// error message when the device is out of memory).  Growing waits for the
+// context's streams first, so no queued kernel still uses the old buffer.
+void *ws_get(Ctx *c, int id, size_t bytes);
+// Uncached buffer of at least `bytes` bytes in `b` (0 or a negative code); a
+// grown buffer replaces the old one, which stays allocated until the process
+// ends (uncached memory is not freed once a kernel has read it).
+int ws_uncached(Ctx *c, DevBuf &b, size_t bytes);
 void prof_begin(Ctx *c, hipStream_t s, const char *name, double flops, double bytes, ProfRec *r);
 void prof_end(Ctx *c, hipStream_t s, ProfRec *r);
 
@@ -109,6 +145,14 @@ template <class T> int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int 
 template <class T>
 int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, const T *W, int NRHS, T *dB,
                  int ldb);
+// Before the first trsm_enqueue of a blocking call: clears the abort word of
+// the wavefront solve on s; after synchronizing: -5 if a hand-off wait gave
+// up (results invalid), else 0.
+int trsm_begin(Ctx *c, hipStream_t s);
+int trsm_status(Ctx *c);
+// FP32 tile factorization (mi_spotrf_tiles) with the trailing-update engine
+// given per call: bf16_split = BF16 MFMA on the two-term split.
+int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split);
 // R = B - A X for the symmetric A in lower tiles, FP64, enqueued on s
 // (mi_mixed.hip); rowabs (N values, may be null) receives the row sums of |A|.
 int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS, const double *dX, int ldx,

@@ -54,6 +54,33 @@ __global__ __launch_bounds__(256) void k_d2s(const double *__restrict__ in, floa
     if (bad) *ovf = 1;
 }
 
+// Packed lower tiles (N, NB): D = float(S) on the lower triangle within N,
+// 0 on the strict upper parts of the diagonal tiles and on padding rows, and
+// *ovf = 1 when a value of the lower triangle exceeds FLT_MAX in magnitude
+// (dlat2s semantics: only the referenced triangle is converted and tested).
+__global__ __launch_bounds__(256) void k_d2s_lower(const double *__restrict__ S, float *__restrict__ D, int N,
+                                                  int NB, int MT, int64_t n, int *__restrict__ ovf)
+{
+    bool bad = false;
+    const int64_t T2 = (int64_t)NB * NB;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t = i / T2;
+        const int e = (int)(i - t * T2), r = e % NB, cl = e / NB;
+        int col = 0;                       // tile column of tile t
+        while (t >= MT - col) { t -= MT - col; col++; }
+        const int row = col + (int)t;
+        const bool ref = row * NB + r < N && col * NB + cl < N && (row != col || r >= cl);
+        float v = 0.f;
+        if (ref) {
+            const double x = S[i];
+            bad |= fabs(x) > (double)FLT_MAX;
+            v = (float)x;
+        }
+        D[i] = v;
+    }
+    if (bad) *ovf = 1;
+}
+
 // column-major blocks: D (ldd) = float(S (lds)) for M x Nc
 __global__ __launch_bounds__(256) void k_d2s_cm(const double *__restrict__ S, int lds, float *__restrict__ D,
                                                int ldd, int M, int Nc, int *__restrict__ ovf)
@@ -214,7 +241,6 @@ __global__ __launch_bounds__(256) void k_dsymm_resid(const double *__restrict__ 
 // row, in a fixed order, so the result is deterministic.
 // Partials per tile: P1 [NB][16], P2 [NB][16], |A| row sums [NB], column
 // sums [NB] -- 34 NB doubles at tile_index(m, n) * 34 NB.
-constexpr int SY_TS = 17;      // column stride of the per-wave transpose buffer (doubles)
 constexpr int SY_PART = 34;    // partial doubles per tile row index
 
 template <bool ABS>
@@ -448,25 +474,6 @@ __global__ void k_flag_clear(int *p) { *p = 0; }
         }                                                                         \
     } while (0)
 
-// Workspaces of the mixed solve, kept across calls (grown, never shrunk) so
-// that a repeated solve allocates nothing: FP32 tiles, FP32 right-hand sides,
-// FP64 residual + norms + row sums, FP32 diagonal-block inverses.
-struct Buf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    void *get(size_t need)
-    {
-        if (bytes >= need) return p;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        if (hipMalloc(&p, need) != hipSuccess) { p = nullptr; return nullptr; }
-        bytes = need;
-        return p;
-    }
-};
-Buf g_l32, g_r32, g_r64, g_w32;
-Buf g_part;      // per-tile partials of the symmetric residual
 
 }  // namespace
 
@@ -481,11 +488,8 @@ int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS
     // loads need an even NB (16-byte aligned pairs)
     if (NB >= 64 && NB <= 512 && NB % 2 == 0) {
         const int64_t ntile = (int64_t)MT * (MT + 1) / 2;
-        double *P = (double *)g_part.get((size_t)ntile * SY_PART * NB * sizeof(double));
-        if (!P) {
-            set_error("out of device memory for the residual partials");
-            return -3;
-        }
+        double *P = (double *)ws_get(c, WS_MX_PART, (size_t)ntile * SY_PART * NB * sizeof(double));
+        if (!P) return -3;
         for (int j0 = 0; j0 < NRHS; j0 += 16) {
             const int nr = NRHS - j0 < 16 ? NRHS - j0 : 16;
             double *ra = j0 == 0 ? rowabs : nullptr;
@@ -558,19 +562,20 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
     const double eps = 0x1p-53;
     int rc = 0;
 
-    float *L32 = (float *)g_l32.get((size_t)ntile_el * sizeof(float));
-    float *R32 = (float *)g_r32.get((size_t)N * NRHS * sizeof(float));
-    double *R = (double *)g_r64.get(((size_t)N * NRHS + 2 * NRHS + 1 + (size_t)N) * sizeof(double));
-    float *W32 = (float *)g_w32.get(trsm_ws_elems(N, NB) * sizeof(float));
-    if (!L32 || !R32 || !R || !W32) {
-        set_error("out of device memory for the mixed-precision solve");
-        return -3;
-    }
+    // workspaces of the context, kept across calls so that a repeated solve
+    // allocates nothing: FP32 tiles, FP32 right-hand sides, FP64 residual +
+    // norms + row sums, FP32 solve workspace
+    float *L32 = (float *)ws_get(c, WS_MX_L32, (size_t)ntile_el * sizeof(float));
+    float *R32 = L32 ? (float *)ws_get(c, WS_MX_R32, (size_t)N * NRHS * sizeof(float)) : nullptr;
+    double *R = R32 ? (double *)ws_get(c, WS_MX_R64, ((size_t)N * NRHS + 2 * NRHS + 1 + (size_t)N) * sizeof(double))
+                    : nullptr;
+    float *W32 = R ? (float *)ws_get(c, WS_MX_W32, trsm_ws_elems(N, NB) * sizeof(float)) : nullptr;
+    if (!W32) return -3;
     double *nrm = R + (size_t)N * NRHS;           // [0, NRHS): ||r_j||, [NRHS, 2 NRHS): ||x_j||, ||A||
     double *rs = nrm + 2 * NRHS + 1;              // row sums of |A|
     std::vector<double> h(2 * NRHS + 1);
     double anrm = 0.0, best = INFINITY;
-    int stall = 0, ovf = 0;
+    int stall = 0, ovf = 0, h_ovf = 0;
     auto fall_back = [&](int code) -> int {
         *iter = code;
         int r = mi_dpotrf_tiles(c, N, NB, dA, info);
@@ -583,14 +588,16 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
 
     // span on the host clock: the nested blocking calls use the context's events
     const auto t_start = std::chrono::steady_clock::now();
-    // A -> FP32 tiles
-    if ((rc = mi_dlag2s_tiles(c, ntile_el, dA, L32, &ovf))) goto done;
+    // A -> FP32 tiles (the lower triangle; the rest of the image is zeroed)
+    k_flag_clear<<<1, 1, 0, s>>>(c->d_info);
+    MI_LAUNCH(c, s, "dlag2s", 0.0, 12.0 * (double)ntile_el,
+              k_d2s_lower<<<grid_of(ntile_el), 256, 0, s>>>(dA, L32, N, NB, MT, ntile_el, c->d_info));
+    MX_CHECK(hipMemcpyAsync(&ovf, c->d_info, sizeof(int), hipMemcpyDeviceToHost, s), done);
+    MX_CHECK(hipStreamSynchronize(s), done);
     if (ovf) { rc = fall_back(-2); goto done; }
     {
         int sinfo = 0;
-        mi_set_spotrf_update(prec);
-        rc = mi_spotrf_tiles(c, N, NB, L32, &sinfo);
-        mi_set_spotrf_update(0);
+        rc = spotrf_tiles_split(c, N, NB, L32, &sinfo, prec == 1);
         if (rc) goto done;
         if (sinfo) { rc = fall_back(-3); goto done; }
     }
@@ -603,8 +610,10 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
         double cte = 0.0;                         // ||A||inf eps sqrt(N), from the first residual
         int it = 0;
         bool conv = false;
+        if ((rc = trsm_begin(c, s))) goto done;
         if ((rc = trsm_prepare<float>(c, s, 0, N, NB, L32, W32))) goto done;
-        // X0 = L32^-T L32^-1 B
+        // X0 = L32^-T L32^-1 B; the overflow flag covers the rounding of B
+        // here and of every residual below (dlag2s: ITER = -2)
         k_flag_clear<<<1, 1, 0, s>>>(c->d_info);
         k_d2s_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(dB, ldb, R32, N, N, NRHS, c->d_info);
         if ((rc = solve32())) goto done;
@@ -618,7 +627,10 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
             if (it == 0) k_colamax<<<1, 256, 0, s>>>(rs, N, N, nrm + 2 * NRHS);
             MX_CHECK(hipMemcpyAsync(h.data(), nrm, (2 * NRHS + 1) * sizeof(double), hipMemcpyDeviceToHost, s),
                      done);
+            MX_CHECK(hipMemcpyAsync(&h_ovf, c->d_info, sizeof(int), hipMemcpyDeviceToHost, s), done);
             MX_CHECK(hipStreamSynchronize(s), done);
+            if ((rc = trsm_status(c))) goto done;
+            if (h_ovf) break;                     // B or a residual overflowed FP32
             if (it == 0) {
                 anrm = h[2 * NRHS];
                 cte = anrm * eps * std::sqrt((double)N);
@@ -626,8 +638,10 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
             double worst = 0.0;
             conv = true;
             for (int j = 0; j < NRHS; j++) {
-                if (!(h[j] <= h[NRHS + j] * cte)) conv = false;
-                worst = (h[j] != h[j]) ? INFINITY : std::max(worst, h[j]);
+                // non-finite norms never count as converged (inf <= inf)
+                if (!(std::isfinite(h[j]) && std::isfinite(h[NRHS + j]) && h[j] <= h[NRHS + j] * cte))
+                    conv = false;
+                worst = std::isfinite(h[j]) ? std::max(worst, h[j]) : INFINITY;
             }
             if (conv || it == ITERMAX) break;
             // stall: the largest residual failed to halve twice in a row
@@ -638,7 +652,9 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
             if ((rc = solve32())) goto done;
             k_s2d_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R32, N, dX, ldx, N, NRHS, 1);
         }
-        if (conv) {
+        if (h_ovf) {
+            rc = fall_back(-2);
+        } else if (conv) {
             *iter = it;
         } else {
             rc = fall_back(it >= ITERMAX ? -31 : -4);

@@ -1122,7 +1122,9 @@ __global__ void k_fill_int(int *p, int v, int n)
 int g_panel_kernel = 1;
 // trailing-update engine: 0 = FP64 VALU, 1 = MFMA (default)
 int g_update_kernel = -1;
-// FP32 trailing update: 0 = FP32 MFMA (default), 1 = BF16 MFMA, two-term split
+// FP32 trailing update of mi_spotrf_tiles and the FP32 step launches: 0 =
+// FP32 MFMA (default), 1 = BF16 MFMA, two-term split (the mixed solve passes
+// its choice as an argument instead)
 int g_supdate = 0;
 
 // profiler names per precision
@@ -1205,7 +1207,7 @@ int launch_trsm(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, cons
 // through the last column).
 template <class T>
 int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin,
-                      int m_off, int64_t ntiles)
+                      int m_off, int64_t ntiles, bool bf16_split)
 {
     if (n_begin >= MT || n_begin + m_off >= MT) return 0;
     int64_t avail = MT - n_begin - m_off;
@@ -1221,7 +1223,7 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
         g_update_kernel = (e && !strcmp(e, "valu")) ? 0 : 1;
     }
     const bool valu = std::is_same<T, double>::value && g_update_kernel == 0;
-    const bool bf16 = std::is_same<T, float>::value && g_supdate == 1;
+    const bool bf16 = std::is_same<T, float>::value && bf16_split;
     const bool small = grid < 512 && !valu && !bf16;
     if (small) {
         nsub = (NB + 63) / 64;
@@ -1255,12 +1257,13 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
 }
 
 template <class T>
-int launch_update(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin, int n_end)
+int launch_update(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin, int n_end,
+                  bool bf16_split)
 {
     if (n_begin >= n_end) return 0;
     int64_t ntiles = 0;
     for (int n = n_begin; n < n_end; n++) ntiles += MT - n;
-    return launch_update_run<T>(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles);
+    return launch_update_run<T>(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles, bf16_split);
 }
 
 template <class T>
@@ -1284,23 +1287,9 @@ int prepare_potrf(int NB)
     return 0;
 }
 
-// diagonal-block inverses handed from POTRF(k) to TRSM(k): two steps in flight
-struct Workspace {
-    void *winv = nullptr;
-    size_t bytes = 0;
-};
-Workspace g_ws;
-
-int ensure_ws(size_t bytes)
-{
-    if (g_ws.bytes >= bytes) return 0;
-    if (g_ws.winv) (void)hipFree(g_ws.winv);
-    g_ws.winv = nullptr;
-    g_ws.bytes = 0;
-    MI_CHECK(hipMalloc(&g_ws.winv, bytes));
-    g_ws.bytes = bytes;
-    return 0;
-}
+// Elements of the diagonal-block inverses handed from POTRF(k) to TRSM(k),
+// two steps in flight.
+inline size_t potrf_ws_elems(int NB) { return (size_t)2 * ((NB + 31) / 32) * 1024; }
 
 // Whole factorization of the packed lower tiles dT (N, NB), resident, with a
 // two-step look-ahead over two streams (src/zpotrf_L.jdf's priorities give
@@ -1314,8 +1303,10 @@ int ensure_ws(size_t bytes)
 // Every tile gets its updates in step order and no two launches touch the
 // same tile at once; the panel chain waits for one column of the bulk of
 // step k-1, not for all of it.
+// bf16_split: FP32 trailing updates on BF16 MFMA with the two-term split.
+// ws: potrf_ws_elems(NB) elements of workspace, or nullptr for the context's.
 template <class T>
-int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info)
+int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws)
 {
     *info = 0;
     if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
@@ -1323,10 +1314,8 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info)
     const int MT = (N + NB - 1) / NB;
     int rc = prepare_potrf<T>(NB);
     if (rc) return rc;
-    const int nblk = (NB + 31) / 32;
-    const size_t wsz = (size_t)nblk * 1024;        // elements per step
-    if ((rc = ensure_ws(2 * wsz * sizeof(T)))) return rc;
-    T *ws = (T *)g_ws.winv;
+    const size_t wsz = potrf_ws_elems(NB) / 2;      // elements per step
+    if (!ws && !(ws = (T *)ws_get(c, WS_POTRF_WINV, potrf_ws_elems(NB) * sizeof(T)))) return -3;
     c->ev_next = 0;
 
     MI_CHECK(hipEventRecord(c->t0, c->s_main));
@@ -1345,7 +1334,7 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info)
     for (int k = 0; k < MT - 1; k++) {
         // panel stream: column k+1 by step k, then the next panel
         if (col_ready) MI_CHECK(hipStreamWaitEvent(c->s_panel, col_ready, 0));
-        if ((rc = launch_update<T>(c, c->s_panel, dT, N, NB, MT, k, k + 1, k + 2))) return rc;
+        if ((rc = launch_update<T>(c, c->s_panel, dT, N, NB, MT, k, k + 1, k + 2, bf16_split))) return rc;
         T *w = ws + ((k + 1) & 1) * wsz;
         if ((rc = launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
         if ((rc = launch_trsm<T>(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
@@ -1355,11 +1344,11 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info)
         MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
         col_ready = nullptr;
         if (k + 2 < MT) {
-            if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 2, k + 3))) return rc;
+            if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 2, k + 3, bf16_split))) return rc;
             col_ready = next_event(c);
             MI_CHECK(hipEventRecord(col_ready, c->s_main));
         }
-        if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 3, MT))) return rc;
+        if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 3, MT, bf16_split))) return rc;
         pk = pk_next;
     }
     MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
@@ -1382,10 +1371,10 @@ int step_potrf(mi_ctx_t *c, int N, int NB, T *dT, int k, int *info)
     const int MT = (N + NB - 1) / NB;
     int rc = prepare_potrf<T>(NB);
     if (rc) return rc;
-    const int nblk = (NB + 31) / 32;
-    if ((rc = ensure_ws((size_t)2 * nblk * 1024 * sizeof(T)))) return rc;
+    T *ws = (T *)ws_get(c, WS_POTRF_WINV, potrf_ws_elems(NB) * sizeof(T));
+    if (!ws) return -3;
     k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
-    if ((rc = launch_potrf<T>(c, c->s_main, dT, N, NB, MT, k, (T *)g_ws.winv))) return rc;
+    if ((rc = launch_potrf<T>(c, c->s_main, dT, N, NB, MT, k, ws))) return rc;
     MI_CHECK(hipMemcpyAsync(info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
     return mi_synchronize(c);
 }
@@ -1395,7 +1384,12 @@ int step_trsm(mi_ctx_t *c, int N, int NB, T *dT, int k)
 {
     // the diagonal-block inverses come from the POTRF launch of the same k
     const int MT = (N + NB - 1) / NB;
-    int rc = launch_trsm<T>(c, c->s_main, dT, N, NB, MT, k, (const T *)g_ws.winv);
+    const T *ws = (const T *)c->ws[WS_POTRF_WINV].p;
+    if (!ws || c->ws[WS_POTRF_WINV].bytes < potrf_ws_elems(NB) * sizeof(T)) {
+        set_error("the panel solve of step k needs the POTRF launch of step k first");
+        return -2;
+    }
+    int rc = launch_trsm<T>(c, c->s_main, dT, N, NB, MT, k, ws);
     if (rc) return rc;
     return mi_synchronize(c);
 }
@@ -1405,12 +1399,19 @@ int step_update(mi_ctx_t *c, int N, int NB, T *dT, int k, int n_begin, int n_end
 {
     const int MT = (N + NB - 1) / NB;
     if (k < 0 || n_begin <= k || n_end > MT) { set_error("bad update range"); return -2; }
-    int rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, n_begin, n_end);
+    int rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, n_begin, n_end, g_supdate == 1);
     if (rc) return rc;
     return mi_synchronize(c);
 }
 
 }  // namespace
+
+namespace mi {
+int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split)
+{
+    return potrf_tiles<float>(c, N, NB, dT, info, bf16_split, nullptr);
+}
+}  // namespace mi
 
 extern "C" {
 
@@ -1467,12 +1468,12 @@ int mi_spotrf_step_update(mi_ctx_t *c, int N, int NB, float *dT, int k, int n_be
 
 int mi_dpotrf_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
 {
-    return potrf_tiles<double>(c, N, NB, dT, info);
+    return potrf_tiles<double>(c, N, NB, dT, info, false, nullptr);
 }
 
 int mi_spotrf_tiles(mi_ctx_t *c, int N, int NB, float *dT, int *info)
 {
-    return potrf_tiles<float>(c, N, NB, dT, info);
+    return potrf_tiles<float>(c, N, NB, dT, info, g_supdate == 1, nullptr);
 }
 
 }  // extern "C"

@@ -18,6 +18,52 @@ void set_error(const char *fmt, ...)
     va_end(ap);
 }
 
+void *ws_get(Ctx *c, int id, size_t bytes)
+{
+    DevBuf &b = c->ws[id];
+    if (b.p && b.bytes >= bytes) return b.p;
+    if (b.p) {
+        // the old buffer may still be read by work queued on this context
+        if (hipStreamSynchronize(c->s_panel) != hipSuccess || hipStreamSynchronize(c->s_main) != hipSuccess) {
+            set_error("workspace %d: stream synchronization failed", id);
+            return nullptr;
+        }
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory: workspace %d of %zu bytes (%s)", id, bytes, hipGetErrorString(e));
+        return nullptr;
+    }
+    b.p = p;
+    b.bytes = bytes;
+    return p;
+}
+
+// Uncached buffers replaced by larger ones or left by destroyed contexts:
+// kept allocated until the process ends.
+static std::mutex g_retired_mu;
+static std::vector<void *> g_retired;
+
+int ws_uncached(Ctx *c, DevBuf &b, size_t bytes)
+{
+    (void)c;
+    if (b.p && b.bytes >= bytes) return 0;
+    void *p = nullptr;
+    MI_CHECK(hipExtMallocWithFlags(&p, bytes, hipDeviceMallocUncached));
+    if (b.p) {
+        std::lock_guard<std::mutex> g(g_retired_mu);
+        g_retired.push_back(b.p);
+    }
+    b.p = p;
+    b.bytes = bytes;
+    return 0;
+}
+
 hipEvent_t next_event(Ctx *c)
 {
     if (c->ev_next >= c->ev_pool.size()) {
@@ -132,6 +178,13 @@ void mi_ctx_destroy(mi_ctx_t *c)
     (void)hipEventDestroy(c->t1);
     (void)hipFree(c->d_info);
     (void)hipFree(c->d_scratch);
+    for (auto &b : c->ws)
+        if (b.p) (void)hipFree(b.p);
+    {
+        std::lock_guard<std::mutex> g(mi::g_retired_mu);
+        if (c->wv_ticket.p) mi::g_retired.push_back(c->wv_ticket.p);
+        if (c->wv_stage.p) mi::g_retired.push_back(c->wv_stage.p);
+    }
     (void)hipStreamDestroy(c->s_main);
     (void)hipStreamDestroy(c->s_panel);
     delete c;

@@ -632,11 +632,16 @@ template <class T, bool TRANS> __device__ inline int wv_xs(int R, int j)
     return ((hb * 4 + (kq >> 1)) * 64 + 16 * f + j) * 2 + (kq & 1);
 }
 
+// Every wait is bounded: a consumer that has polled for more than
+// wait_ticks of the 100 MHz real-time counter (0: gives up at once) stores 1
+// into the abort word flags[1] and goes on with whatever it read; every other
+// wait ends as soon as it sees that word set, so the launch always drains and
+// the host reports the failure.
 template <class T, bool TRANS>
 __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, int N, int NB, int MT,
                                                      const T *__restrict__ W128, const T *__restrict__ Mlink,
                                                      T *__restrict__ B, int ldb, int nr, T *__restrict__ xs,
-                                                     int *__restrict__ ticket)
+                                                     int *__restrict__ ticket, long long wait_ticks)
 {
     typedef typename MF<T>::v4 V4;
     typedef typename MF<T>::v2 V2;
@@ -757,11 +762,28 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
     // last block (fin) lanes 0..3 first poll one value of each producer
     // wave; then the wave loads its values, and loads again while any of
     // them is still the sentinel
+    // true once this wait has to end without its values: the abort word is
+    // set, or this wait outlived its budget (then it sets the word)
+    int *abort_word = ticket + 1;
+    auto give_up = [&](long long t_end) -> bool {
+        const int *ap = abort_word;
+        asm volatile("" : "+v"(ap));
+        if (__hip_atomic_load(ap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return true;
+        if (wait_ticks > 0 && (long long)__builtin_amdgcn_s_memrealtime() < t_end) return false;
+        int *wp = abort_word;
+        asm volatile("" : "+v"(wp));
+        __hip_atomic_store(wp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    };
     auto wait_x = [&](int h, bool fin, T (&bx)[4][8]) {
         const T *xh = xs + (int64_t)h * (WV_RB * RHSB);
+        const long long t_end = (long long)__builtin_amdgcn_s_memrealtime() + wait_ticks;
+        if (wait_ticks == 0) (void)give_up(t_end);   // a zero budget ends every wait at once
         if (fin && lane < 4)
-            while (wv_load(xh + wv_xs<T, TRANS>(32 * lane + 31, RHSB - 1)) == WvSent<T>::v)
+            while (wv_load(xh + wv_xs<T, TRANS>(32 * lane + 31, RHSB - 1)) == WvSent<T>::v) {
+                if (give_up(t_end)) break;
                 __builtin_amdgcn_s_sleep(1);
+            }
         for (;;) {
             int ok = 1;
             if constexpr (sizeof(T) == 8) {
@@ -789,6 +811,7 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
                     }
             }
             if (__all(ok)) break;
+            if (__any(give_up(t_end))) break;
             __builtin_amdgcn_s_sleep(1);
         }
 #pragma unroll
@@ -856,7 +879,8 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
         }
 }
 
-// Before each wavefront launch: staging filled with the sentinel, ticket 0.
+// Before each wavefront launch: staging filled with the sentinel, ticket 0
+// (the abort word flags[1] is cleared once per solve, by k_wave_abort_clear).
 template <class T>
 __global__ void k_wave_reset(T *__restrict__ xs, int64_t n, int *__restrict__ ticket)
 {
@@ -881,40 +905,21 @@ __global__ void k_scale(T *__restrict__ B, int N, int nrhs, int ldb, T alpha)
     }
 }
 
-struct SolveWs {
-    void *w = nullptr;
-    size_t bytes = 0;
-};
-SolveWs g_sws;
-
-int ensure_sws(size_t bytes)
+__global__ void k_wave_abort_clear(int *flags)
 {
-    if (g_sws.bytes >= bytes) return 0;
-    if (g_sws.w) (void)hipFree(g_sws.w);
-    g_sws.w = nullptr;
-    g_sws.bytes = 0;
-    MI_CHECK(hipMalloc(&g_sws.w, bytes));
-    g_sws.bytes = bytes;
-    return 0;
+    int *wp = flags + 1;
+    asm volatile("" : "+v"(wp));           // vector store
+    *wp = 0;
 }
 
-// Hand-off memory of the wavefront solve: the ticket and the staging copy
-// of X, in uncached device memory so that every access reaches
-// memory and a value stored by one XCD is visible to the others without cache
-// maintenance.  Uncached allocations are never freed once a kernel has read
-// them (a grown buffer replaces the old one, which stays allocated).
-SolveWs g_wflags, g_wstage;
+// The hand-off memory of the wavefront solve (the ticket and abort word, and
+// the staging copy of X) is the context's uncached pair wv_ticket / wv_stage:
+// every access reaches memory, so a value stored by one XCD is visible to
+// the others without cache maintenance.
 int g_wave_solve = 1;   // 1: one-launch wavefront solve when NB % 32 == 0; 0: per-tile launches
-
-int ensure_uncached(SolveWs &w, size_t bytes)
-{
-    if (w.bytes >= bytes) return 0;
-    void *p = nullptr;
-    MI_CHECK(hipExtMallocWithFlags(&p, bytes, hipDeviceMallocUncached));
-    w.w = p;
-    w.bytes = bytes;
-    return 0;
-}
+// bound on one hand-off wait of the wavefront solve, in ticks of the 100 MHz
+// real-time counter (default 10 s; 0 = give up at once)
+long long g_wave_wait_ticks = 1000000000LL;
 
 int set_lds(const void *f, size_t bytes)
 {
@@ -1042,10 +1047,10 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
         // one wavefront launch per chunk of 16 right-hand sides
         const int nwg = (N + WV_RB - 1) / WV_RB;
         int rc;
-        if ((rc = ensure_uncached(g_wflags, sizeof(int)))) return rc;
-        if ((rc = ensure_uncached(g_wstage, (size_t)nwg * WV_RB * RHSB * sizeof(T)))) return rc;
-        int *ticket = (int *)g_wflags.w;
-        T *xs = (T *)g_wstage.w;
+        if ((rc = ws_uncached(c, c->wv_ticket, 2 * sizeof(int)))) return rc;
+        if ((rc = ws_uncached(c, c->wv_stage, (size_t)nwg * WV_RB * RHSB * sizeof(T)))) return rc;
+        int *ticket = (int *)c->wv_ticket.p;
+        T *xs = (T *)c->wv_stage.p;
         const T *W128 = W + (size_t)MT * ((NB + 31) / 32) * 1024;
         const T *Mf = W128 + (size_t)nwg * 16384, *Mb = Mf + (size_t)nwg * 16384;
         const int64_t nst = (int64_t)nwg * WV_RB * RHSB;
@@ -1055,8 +1060,10 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
             k_wave_reset<T><<<(unsigned)std::min<int64_t>((nst + 255) / 256, 1024), 256, 0, s>>>(xs, nst, ticket);
             MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
                       (trans == 111
-                       ? k_trsm_wave<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs, ticket)
-                       : k_trsm_wave<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs, ticket)));
+                       ? k_trsm_wave<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs, ticket,
+                                                                   g_wave_wait_ticks)
+                       : k_trsm_wave<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs, ticket,
+                                                                  g_wave_wait_ticks)));
         }
         return 0;
     }
@@ -1067,6 +1074,31 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
         int rc = (nr == 1 && NB <= TSV_THREADS) ? solve_chunk<T, 1>(c, s, trans, N, NB, MT, dT, W, Bj, ldb, nr)
                                                 : solve_chunk<T, RHSB>(c, s, trans, N, NB, MT, dT, W, Bj, ldb, nr);
         if (rc) return rc;
+    }
+    return 0;
+}
+
+// Clears the abort word of the wavefront solve on s (before the first
+// trsm_enqueue of a blocking call).
+int trsm_begin(Ctx *c, hipStream_t s)
+{
+    int rc;
+    if ((rc = ws_uncached(c, c->wv_ticket, 2 * sizeof(int)))) return rc;
+    k_wave_abort_clear<<<1, 1, 0, s>>>((int *)c->wv_ticket.p);
+    MI_CHECK(hipGetLastError());
+    return 0;
+}
+
+// After the context's streams are synchronized: -5 when a wait of the
+// wavefront solve gave up (its results are not valid), else 0.
+int trsm_status(Ctx *c)
+{
+    int ab = 0;
+    if (!c->wv_ticket.p) return 0;
+    MI_CHECK(hipMemcpy(&ab, (int *)c->wv_ticket.p + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (ab) {
+        set_error("triangular solve: a hand-off wait between workgroups exceeded its time limit");
+        return -5;
     }
     return 0;
 }
@@ -1082,16 +1114,21 @@ template int trsm_enqueue<float>(Ctx *, hipStream_t, int, int, int, const float 
 
 namespace {
 
+// B <- alpha op_last(L)^{-1} ... op_first(L)^{-1} B for the nt sweeps
+// trans[0..nt) (111 NoTrans / 112 Trans), with one preparation of the block
+// inverses for all of them; blocking.
 template <class T>
-int trsm_LL(mi_ctx_t *c, int trans, int diag, T alpha, int N, int NB, const T *dT, int NRHS, T *dB, int ldb)
+int trsm_run(mi_ctx_t *c, int nt, const int *trans, int diag, T alpha, int N, int NB, const T *dT, int NRHS,
+             T *dB, int ldb)
 {
     if (N <= 0 || NRHS <= 0) return 0;
     if (NB <= 0 || ldb < N) { set_error("bad sizes"); return -2; }
     if (NB > TSV_MAX_NB) { set_error("tile size NB=%d above the solve limit %d", NB, TSV_MAX_NB); return -4; }
     int rc;
-    if ((rc = ensure_sws(trsm_ws_elems(N, NB) * sizeof(T)))) return rc;
-    T *W = (T *)g_sws.w;
+    T *W = (T *)ws_get(c, WS_TRSM, trsm_ws_elems(N, NB) * sizeof(T));
+    if (!W) return -3;
     hipStream_t s = c->s_main;
+    if ((rc = trsm_begin(c, s))) return rc;
     MI_CHECK(hipEventRecord(c->t0, s));
     if (alpha != T(1)) {
         const int64_t tot = (int64_t)N * NRHS;
@@ -1099,9 +1136,11 @@ int trsm_LL(mi_ctx_t *c, int trans, int diag, T alpha, int N, int NB, const T *d
                   k_scale<T><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(dB, N, NRHS, ldb, alpha));
     }
     if ((rc = trsm_prepare<T>(c, s, diag == 132, N, NB, dT, W))) return rc;
-    if ((rc = trsm_enqueue<T>(c, s, trans, N, NB, dT, W, NRHS, dB, ldb))) return rc;
+    for (int i = 0; i < nt; i++)
+        if ((rc = trsm_enqueue<T>(c, s, trans[i], N, NB, dT, W, NRHS, dB, ldb))) return rc;
     MI_CHECK(hipEventRecord(c->t1, s));
     if ((rc = mi_synchronize(c))) return rc;
+    if ((rc = trsm_status(c))) return rc;
     float ms = 0.f;
     MI_CHECK(hipEventElapsedTime(&ms, c->t0, c->t1));
     c->last_ms = ms;
@@ -1109,14 +1148,17 @@ int trsm_LL(mi_ctx_t *c, int trans, int diag, T alpha, int N, int NB, const T *d
 }
 
 template <class T>
+int trsm_LL(mi_ctx_t *c, int trans, int diag, T alpha, int N, int NB, const T *dT, int NRHS, T *dB, int ldb)
+{
+    return trsm_run<T>(c, 1, &trans, diag, alpha, N, NB, dT, NRHS, dB, ldb);
+}
+
+// L^{-T} L^{-1} B: both sweeps on one preparation (src/zpotrs_wrapper.c:91-94)
+template <class T>
 int potrs_tiles(mi_ctx_t *c, int N, int NB, const T *dT, int NRHS, T *dB, int ldb)
 {
-    int rc = trsm_LL<T>(c, 111, 131, T(1), N, NB, dT, NRHS, dB, ldb);
-    if (rc) return rc;
-    const double ms = c->last_ms;
-    rc = trsm_LL<T>(c, 112, 131, T(1), N, NB, dT, NRHS, dB, ldb);
-    c->last_ms += ms;
-    return rc;
+    const int tr[2] = {111, 112};
+    return trsm_run<T>(c, 2, tr, 131, T(1), N, NB, dT, NRHS, dB, ldb);
 }
 
 }  // namespace
@@ -1145,6 +1187,13 @@ extern "C" int mi_spotrs_tiles(mi_ctx_t *c, int N, int NB, const float *dT, int 
 
 // 1 (default): the one-launch wavefront solve whenever NB % 32 == 0;
 // 0: one diagonal-solve and one update launch per tile.  Process-wide.
+extern "C" int mi_set_solve_wait_limit(double seconds)
+{
+    if (!(seconds >= 0.0) || seconds > 3600.0) { set_error("wait limit must be in [0, 3600] s"); return -2; }
+    g_wave_wait_ticks = (long long)(seconds * 1e8);
+    return 0;
+}
+
 extern "C" int mi_set_solve_engine(int which)
 {
     if (which != 0 && which != 1) { set_error("solve engine must be 0 or 1"); return -2; }

@@ -151,11 +151,15 @@ int check_dpotrf(parsec_context_t *parsec, int loud, dplasma_enum_t uplo, parsec
     if (dpl_sym_acquire(parsec, A, uplo, 1, &sl, NULL)) return 1;
     if (dpl_sym_acquire(parsec, A0, uplo, 1, &sa, NULL)) { dpl_sym_release(parsec, A, &sl, 0, NULL); return 1; }
     double Anorm = 0.0, Rnorm = 0.0;
+    memset(&parsec->last_check, 0, sizeof parsec->last_check);
     int rc = mi_dcheck_potrf(parsec->mi, sl.N, sl.NB, sl.d, sa.d, &Anorm, &Rnorm);
     dpl_sym_release(parsec, A0, &sa, 0, NULL);
     dpl_sym_release(parsec, A, &sl, 0, NULL);
     if (rc) { dpl_mi_fail("check_dpotrf", rc); return 1; }
     double result = Rnorm / (Anorm * N * eps_d());
+    parsec->last_check.Anorm = Anorm;
+    parsec->last_check.Rnorm = Rnorm;
+    parsec->last_check.result = result;
     if (loud > 2) {
         printf("============\n");
         printf("Checking the Cholesky factorization \n");
@@ -179,11 +183,16 @@ int check_dpoinv(parsec_context_t *parsec, int loud, dplasma_enum_t uplo, parsec
     if (dpl_sym_acquire(parsec, A, uplo, 1, &sa, NULL)) return 1;
     if (dpl_sym_acquire(parsec, Ainv, uplo, 1, &si, NULL)) { dpl_sym_release(parsec, A, &sa, 0, NULL); return 1; }
     double Anorm = 0.0, Ainvnorm = 0.0, Rnorm = 0.0;
+    memset(&parsec->last_check, 0, sizeof parsec->last_check);
     int rc = mi_dcheck_poinv(parsec->mi, sa.N, sa.NB, sa.d, si.d, &Anorm, &Ainvnorm, &Rnorm);
     dpl_sym_release(parsec, Ainv, &si, 0, NULL);
     dpl_sym_release(parsec, A, &sa, 0, NULL);
     if (rc) { dpl_mi_fail("check_dpoinv", rc); return 1; }
     double result = Rnorm / ((Anorm * Ainvnorm) * N * eps_d());
+    parsec->last_check.Anorm = Anorm;
+    parsec->last_check.Ainvnorm = Ainvnorm;
+    parsec->last_check.Rnorm = Rnorm;
+    parsec->last_check.result = result;
     if (loud > 2)
         printf("  ||A||_one = %e, ||A^(-1)||_one = %e, ||I - A * A^(-1)||_one = %e, result = %e\n", Anorm,
                Ainvnorm, Rnorm, result);
@@ -204,12 +213,18 @@ int check_daxmb(parsec_context_t *parsec, int loud, dplasma_enum_t uplo, parsec_
         return 1;
     }
     double An = 0, Bn = 0, Xn = 0, Rn = 0;
+    memset(&parsec->last_check, 0, sizeof parsec->last_check);
     int rc = mi_dcheck_axmb(parsec->mi, sa.N, sa.NB, sa.d, gb.N, gb.d, gb.ld, gx.d, gx.ld, &An, &Bn, &Xn, &Rn);
     dpl_ge_release(parsec, x, &gx, 0, NULL);
     dpl_ge_release(parsec, b, &gb, 0, NULL);
     dpl_sym_release(parsec, A, &sa, 0, NULL);
     if (rc) { dpl_mi_fail("check_daxmb", rc); return 1; }
     double result = Rn / ((An * Xn + Bn) * N * eps_d());
+    parsec->last_check.Anorm = An;
+    parsec->last_check.Bnorm = Bn;
+    parsec->last_check.Xnorm = Xn;
+    parsec->last_check.Rnorm = Rn;
+    parsec->last_check.result = result;
     if (loud > 2) {
         printf("============\n");
         printf("Checking the Residual of the solution \n");
@@ -239,6 +254,13 @@ int dplasma_data_sync_host(parsec_context_t *parsec, parsec_tiled_matrix_t *A)
 int dplasma_data_release(parsec_context_t *parsec, parsec_tiled_matrix_t *A)
 {
     return dpl_release(parsec, A);
+}
+
+int dplasma_mi_last_check(parsec_context_t *parsec, dplasma_mi_check_t *c)
+{
+    if (!parsec || !c) return -1;
+    *c = parsec->last_check;
+    return 0;
 }
 
 int dplasma_mi_last_timing(parsec_context_t *parsec, dplasma_mi_timing_t *t)

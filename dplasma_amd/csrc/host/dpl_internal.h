@@ -38,6 +38,7 @@ struct parsec_context_s {
     parsec_taskpool_t *pending[DPL_MAX_PENDING];
     int npending;
     dplasma_mi_timing_t last;
+    dplasma_mi_check_t last_check;
 };
 
 /* device image of the uplo part of a matrix: packed lower tiles (see

@@ -187,6 +187,14 @@ int dplasma_dpotrs(parsec_context_t *parsec, dplasma_enum_t uplo, const parsec_t
         dpl_error("dplasma_dpotrs", "illegal value of uplo");
         return -1;
     }
+    /* B must have N rows: the reference's two dplasma_ztrsm calls each
+     * reject this shape with -6 and leave B untouched (ztrsm_wrapper.c:
+     * 281-285), but zpotrs_wrapper.c:88-94 drops their codes and returns 0;
+     * here the -6 is returned (B untouched as there). */
+    if (A->m != A->n || B->m != A->n) {
+        dpl_error("dplasma_dpotrs", "illegal matrix A or B (B must have as many rows as A)");
+        return -6;
+    }
     /* L L^T X = B as two triangular solves on one staged image of A
      * (zpotrs_wrapper.c:88-94: Upper = ConjTrans then NoTrans, Lower =
      * NoTrans then ConjTrans; both are L then L^T on the lower image) */
@@ -213,6 +221,13 @@ int dplasma_dposv(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_tiled_ma
     if (uplo != dplasmaUpper && uplo != dplasmaLower) {
         dpl_error("dplasma_dposv", "illegal value of uplo");
         return -1;
+    }
+    /* checked before the factorization: the reference factors A and then
+     * its solves fail with -6 (B untouched) while it returns info; here A
+     * and B are both left untouched and -6 is returned */
+    if (A->m != A->n || B->m != A->n) {
+        dpl_error("dplasma_dposv", "illegal matrix A or B (B must have as many rows as A)");
+        return -6;
     }
     dpl_sym_dev_t s;
     dpl_ge_dev_t g;

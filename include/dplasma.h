@@ -144,6 +144,20 @@ typedef struct dplasma_mi_timing_s {
 } dplasma_mi_timing_t;
 int dplasma_mi_last_timing(parsec_context_t *parsec, dplasma_mi_timing_t *t);
 
+/* Norms behind the last check_dpotrf / check_daxmb / check_dpoinv on this
+ * context (the quantities src/dplasma_zcheck.c:100-104,193-202,290-300
+ * prints), so that a caller can compare them, not only the pass/fail
+ * answer.  Unused fields are 0. */
+typedef struct dplasma_mi_check_s {
+    double Anorm;      /* ||A0||_oo (check_dpotrf, check_daxmb), ||A||_1 (check_dpoinv) */
+    double Rnorm;      /* ||L L^T - A0||_oo, ||b - A0 x||_oo, ||I - Ainv A||_1 */
+    double Bnorm;      /* ||b||_oo (check_daxmb) */
+    double Xnorm;      /* ||x||_oo (check_daxmb) */
+    double Ainvnorm;   /* ||Ainv||_1 (check_dpoinv) */
+    double result;     /* the ratio compared with the threshold */
+} dplasma_mi_check_t;
+int dplasma_mi_last_check(parsec_context_t *parsec, dplasma_mi_check_t *c);
+
 /* The kernel-library context behind a parsec context (for the profiler of
  * dplasma_mi.h). */
 struct mi_ctx_s *dplasma_mi_context(parsec_context_t *parsec);

@@ -14,6 +14,14 @@
  *   column-major        right-hand sides B: N x NRHS, leading dimension ldb.
  *
  * Each hot-path entry cites the reference task body or wrapper it replaces.
+ *
+ * Threading: a context owns its streams and every device workspace its
+ * calls use (grown on demand, freed by mi_ctx_destroy), so different
+ * contexts may be used at the same time from different threads.  Calls on
+ * one context must not overlap (as for the reference, whose calls on one
+ * parsec context are serialized by dplasma_wait_until_completion,
+ * src/dplasmaaux.h:97-101).  The mi_set_* engine switches are process-wide
+ * and must be set while no call is running.  mi_last_error() is per thread.
  */
 #ifndef DPLASMA_MI_KERNELS_H
 #define DPLASMA_MI_KERNELS_H
@@ -118,6 +126,11 @@ int mi_dpotrs_tiles(mi_ctx_t *ctx, int N, int NB, const double *dT, int NRHS, do
  * through uncached device memory) whenever NB % 32 == 0; 0 = one
  * diagonal-solve and one update launch per tile.  Process-wide. */
 int mi_set_solve_engine(int which);
+/* Bound on one hand-off wait of the wavefront solve, in seconds (default
+ * 10; 0 gives up at once).  A wait that exceeds it ends the launch early:
+ * the blocking call then returns -5 and its right-hand sides are not valid.
+ * Process-wide. */
+int mi_set_solve_wait_limit(double seconds);
 /* FP32 tiles and right-hand sides: the "s" precision of the two calls above. */
 int mi_strsm_LL(mi_ctx_t *ctx, int trans, int diag, float alpha, int N, int NB,
                 const float *dT, int NRHS, float *dB, int ldb);

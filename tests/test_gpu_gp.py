@@ -110,4 +110,6 @@ def test_gp_fp32_mixed(ag):
     ag.gp_nll(Kt, yt, NB).backward()
     G = Kt.grad.double().cpu().numpy()
     Ga = oad.gp_nll_grad_K(K, y)
-    assert np.abs(G - Ga).max() <= 2e-2 * np.abs(Ga).max(), np.abs(G - Ga).max() / np.abs(Ga).max()
+    rel = np.abs(G - Ga).max() / np.abs(Ga).max()
+    print(f"FP32 GP gradient, relative error against the FP64 analytic gradient: {rel:.3e}")
+    assert rel <= 2e-3, rel

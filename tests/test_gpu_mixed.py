@@ -80,7 +80,7 @@ def test_dlag2s_overflow(dev):
 @pytest.mark.parametrize("N,NB,nrhs,kappa,prec,expect", [
     (1000, 128, 16, 1e2, 0, "refined"), (1000, 128, 16, 1e5, 0, "refined"), (1000, 128, 16, 1e8, 0, "fallback"),
     (700, 93, 3, 1e3, 0, "refined"), (512, 512, 1, 1e2, 0, "refined"),
-    (1000, 128, 16, 1e2, 1, "refined"), (1000, 128, 16, 1e8, 1, "fallback"),
+    (1000, 128, 16, 1e2, 1, "refined"), (1000, 128, 16, 1e5, 1, "refined"), (1000, 128, 16, 1e8, 1, "fallback"),
 ])
 def test_dsposv(dev, po, N, NB, nrhs, kappa, prec, expect):
     """FP32 factor (prec 0: FP32 MFMA, 1: BF16 split updates) + FP64
