@@ -116,6 +116,13 @@ def libs():
     _sig(M, "mi_dplrnt", i, [vp, i, i, vp, i, i, i, i, u64])
     _sig(M, "mi_dadd_diag", i, [vp, i, vp, i, d])
     _sig(M, "mi_dpotrf_tiles", i, [vp, i, i, vp, pi])
+    _sig(M, "mi_dtd_create", i, [vp, i, i, vp, ctypes.POINTER(vp)])
+    _sig(M, "mi_dtd_potrf", i, [vp, i])
+    _sig(M, "mi_dtd_trsm", i, [vp, i, i])
+    _sig(M, "mi_dtd_syrk", i, [vp, i, i])
+    _sig(M, "mi_dtd_gemm", i, [vp, i, i, i])
+    _sig(M, "mi_dtd_wait", i, [vp, pi, ctypes.POINTER(ctypes.c_long)])
+    _sig(M, "mi_dtd_destroy", None, [vp])
     _sig(M, "mi_dtrsm_LL", i, [vp, i, i, d, i, i, vp, i, vp, i])
     _sig(M, "mi_dpotrs_tiles", i, [vp, i, i, vp, i, vp, i])
     _sig(M, "mi_dpotrf_lapack", i, [vp, i, i, vp, i, i, pi])
@@ -159,6 +166,7 @@ def libs():
     _sig(M, "mi_dpotrf_step_update", i, [vp, i, i, vp, i, i, i])
     _sig(M, "mi_set_update_engine", i, [i])
     _sig(M, "mi_set_panel_engine", i, [i])
+    _sig(M, "mi_set_potrf_server", i, [i])
     _sig(M, "mi_set_spotrf_update", i, [i])
     _sig(M, "mi_set_solve_engine", i, [i])
     _sig(M, "mi_set_solve_wait_limit", i, [d])
@@ -168,6 +176,7 @@ def libs():
     _sig(M, "mi_prof_reset", i, [vp])
     _sig(M, "mi_prof_get", i, [vp, ctypes.c_char_p, pi, pd, pd, pd])
     _sig(M, "mi_prof_count", i, [vp])
+    _sig(M, "mi_prof_busy_ms", i, [vp, ctypes.c_char_p, pd])
     _sig(M, "mi_prof_name", ctypes.c_char_p, [vp, i])
     _sig(M, "mi_last_span_ms", i, [vp, pd])
 
@@ -213,6 +222,22 @@ def libs():
     _sig(D, "dplasma_mi_last_timing", i, [vp, ctypes.POINTER(Timing)])
     _sig(D, "dplasma_mi_last_check", i, [vp, ctypes.POINTER(CheckNorms)])
     _sig(D, "dplasma_mi_context", vp, [vp])
+    _sig(D, "parsec_dtd_taskpool_new", vp, [])
+    _sig(D, "parsec_dtd_data_collection_init", i, [vp])
+    _sig(D, "parsec_dtd_data_collection_fini", None, [vp])
+    _sig(D, "parsec_shim_data_key", ctypes.c_uint64, [vp, i, i])
+    _sig(D, "parsec_dtd_tile_of", vp, [vp, ctypes.c_uint64])
+    _sig(D, "parsec_dtd_create_dpotrf_task_class", vp, [vp, i, i])
+    _sig(D, "parsec_dtd_create_dtrsm_task_class", vp, [vp, i, i])
+    _sig(D, "parsec_dtd_create_dsyrk_task_class", vp, [vp, i, i])
+    _sig(D, "parsec_dtd_create_dgemm_task_class", vp, [vp, i, i])
+    _sig(D, "parsec_dtd_task_class_release", None, [vp, vp])
+    _sig(D, "parsec_dtd_data_flush_all", None, [vp, vp])
+    _sig(D, "parsec_taskpool_wait", i, [vp])
+    _sig(D, "parsec_taskpool_free", None, [vp])
+    _sig(D, "parsec_shim_dtd_ntasks", ctypes.c_long, [vp])
+    # variadic: (tp, tc, priority, device_type, (flag, pointer)..., 0)
+    D.parsec_dtd_insert_task_with_task_class.restype = i
     return _mi, _dpl
 
 
@@ -427,6 +452,14 @@ class Context:
     def prof_reset(self):
         mi, _ = libs()
         mi.mi_prof_reset(self.D.dplasma_mi_context(self.ptr))
+
+    def prof_busy_ms(self, kernel=None):
+        """Union of the launch intervals of `kernel` (all kernels if None), ms."""
+        mi, _ = libs()
+        ms = ctypes.c_double()
+        mi.mi_prof_busy_ms(self.D.dplasma_mi_context(self.ptr), kernel.encode() if kernel else None,
+                           ctypes.byref(ms))
+        return ms.value
 
     def prof(self):
         """{kernel: (launches, total_ms, algorithmic flops, algorithmic bytes)}"""

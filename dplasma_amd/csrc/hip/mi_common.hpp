@@ -77,6 +77,10 @@ struct ProfAgg {
     std::string name;
     int launches = 0;
     double ms = 0.0, flops = 0.0, bytes = 0.0;
+    // [start, end] of every launch, ms after the start event (t0) of the
+    // blocking call it belonged to, and that call's sequence number
+    std::vector<double> t_start, t_end;
+    std::vector<int> call;
 };
 
 // Device workspaces owned by a context: grown on demand (ws_get), freed with
@@ -107,8 +111,10 @@ struct Ctx {
     int device = 0;
     hipStream_t s_main = nullptr;   // trailing updates, solves, copies
     hipStream_t s_panel = nullptr;  // diagonal factorization + panel solve (look-ahead)
+    hipStream_t s_aux = nullptr;    // the persistent diagonal-factor server
     std::vector<hipEvent_t> ev_pool;   // cross-stream dependencies
     size_t ev_next = 0;
+    std::vector<hipEvent_t> ev_ring;   // fixed ring (created with the context)
     int *d_info = nullptr;          // device-side first-failure info
     double *d_scratch = nullptr;    // small reductions (norms)
     size_t scratch_bytes = 0;
@@ -116,12 +122,14 @@ struct Ctx {
     // hand-off memory of the wavefront solve (uncached; an uncached buffer
     // that a kernel has read is never freed, see ws_uncached)
     DevBuf wv_ticket, wv_stage;
+    DevBuf sv_flags;                // flags of the diagonal-factor server (uncached)
     bool prof = false;
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> prof_free;
     std::vector<ProfAgg> agg;
     hipEvent_t t0 = nullptr, t1 = nullptr;  // span of the last blocking call
     double last_ms = 0.0;
+    int prof_calls = 0;             // blocking calls whose launches were collected
 };
 
 void set_error(const char *fmt, ...);
@@ -153,6 +161,17 @@ int trsm_status(Ctx *c);
 // FP32 tile factorization (mi_spotrf_tiles) with the trailing-update engine
 // given per call: bf16_split = BF16 MFMA on the two-term split.
 int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split);
+// bound on the waits of the diagonal-factor server (mi_set_solve_wait_limit)
+void set_server_wait_ticks(long long ticks);
+// Single-tile tasks of the tile Cholesky on the packed lower FP64 image,
+// enqueued on s (mi_potrf.hip): POTRF(k) writes the inverses of its 32 x 32
+// diagonal blocks to Winv (task_winv_elems(NB) elements), which TRSM(m, k)
+// reads; update: A(m, n) -= A(m, k) A(n, k)^T, SYRK when m == n.
+int task_prepare_fp64(int NB);
+int task_potrf_fp64(Ctx *c, hipStream_t s, double *dT, int N, int NB, int k, double *Winv);
+int task_trsm_fp64(Ctx *c, hipStream_t s, double *dT, int N, int NB, int m, int k, const double *Winv);
+int task_update_fp64(Ctx *c, hipStream_t s, double *dT, int N, int NB, int m, int n, int k);
+size_t task_winv_elems(int NB);
 // R = B - A X for the symmetric A in lower tiles, FP64, enqueued on s
 // (mi_mixed.hip); rowabs (N values, may be null) receives the row sums of |A|.
 int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS, const double *dX, int ldx,

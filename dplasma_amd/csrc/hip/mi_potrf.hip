@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include "mi_common.hpp"
 #include "dplasma_mi.h"
@@ -156,9 +157,8 @@ __device__ inline void syrk_unit32(T *__restrict__ A, int ld, int n, const T *__
 // info = k*NB + j of the first non-positive pivot j (1-based), as
 // CORE_dpotrf reports it (src/zpotrf_L.jdf:200-202).
 template <class T>
-__global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, int ld, int RC,
-                                                   int info_base, int *__restrict__ d_info,
-                                                   T *__restrict__ Winv)
+__device__ inline void potrf_tile_dev(T *__restrict__ A, int n, int ld, int RC, int info_base,
+                                      int *__restrict__ d_info, T *__restrict__ Winv)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     typedef typename MF<T>::v4 V4;
@@ -273,6 +273,97 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
     if (tid == 0 && s_fail) atomicCAS(d_info, 0, info_base + s_fail);
 }
 
+template <class T>
+__global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, int ld, int RC,
+                                                   int info_base, int *__restrict__ d_info,
+                                                   T *__restrict__ Winv)
+{
+    potrf_tile_dev<T>(A, n, ld, RC, info_base, d_info, Winv);
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal-factor server.  POTRF(k) needs a whole CU (512 threads, most of
+// the LDS), and inside the factorization a CU only empties once the bulk
+// update of the step has dispatched all its workgroups, so a POTRF launch
+// waits there.  Instead, one persistent workgroup with the same shape holds
+// a CU for the whole factorization and runs POTRF(k0), POTRF(k0+1), ... as
+// the panel stream posts them:
+//   panel stream  ... column k updated -> k_sv_post(k) -> k_sv_wait(k) -> TRSM(k)
+//   server        wait posted >= k; acquire; POTRF(k); release; done = k
+// flags (uncached device memory): [0] last posted step, [1] last finished
+// step, [2] abort.  Every wait is bounded by wait_ticks of the 100 MHz
+// real-time counter: a wait that runs out sets the abort word, and every
+// wait ends once it is set, so all launches drain and the host reports it.
+// Tile data is ordinary device memory, ordered by agent-scope release
+// (server after POTRF; the posting kernel runs after the column update has
+// completed) and acquire fences (server before POTRF, k_sv_wait before TRSM).
+__device__ inline int sv_load(const int *p)
+{
+    asm volatile("" : "+v"(p));
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void sv_store(int *p, int v)
+{
+    asm volatile("" : "+v"(p));
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits (one thread) until flags[idx] >= k; false when the abort word is
+// set or the budget ran out (then it sets the abort word).
+__device__ inline bool sv_wait_ge(int *flags, int idx, int k, long long wait_ticks)
+{
+    const long long t_end = (long long)__builtin_amdgcn_s_memrealtime() + wait_ticks;
+    if (wait_ticks == 0) {                    // a zero budget ends every wait at once
+        sv_store(flags + 2, 1);
+        return false;
+    }
+    for (;;) {
+        if (sv_load(flags + 2)) return false;
+        if (sv_load(flags + idx) >= k) return true;
+        if ((long long)__builtin_amdgcn_s_memrealtime() >= t_end) {
+            sv_store(flags + 2, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(512) void k_potrf_server(T *__restrict__ dT, int N, int NB, int MT, int RC, int k0,
+                                                     int *__restrict__ flags, T *__restrict__ ws, int wsz,
+                                                     int *__restrict__ d_info, long long wait_ticks)
+{
+    __shared__ int s_go;
+    for (int k = k0; k < MT; k++) {
+        if (threadIdx.x == 0) s_go = sv_wait_ge(flags, 0, k, wait_ticks) ? 1 : 0;
+        __syncthreads();
+        if (!s_go) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        T *A = dT + tile_index(MT, k, k) * (int64_t)NB * NB;
+        potrf_tile_dev<T>(A, tile_rows(N, NB, MT, k), NB, RC, k * NB, d_info, ws + (k & 1) * (int64_t)wsz);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) sv_store(flags + 1, k);
+    }
+}
+
+__global__ void k_sv_reset(int *flags)
+{
+    if (threadIdx.x < 3) sv_store(flags + threadIdx.x, threadIdx.x < 2 ? -1 : 0);
+}
+
+__global__ void k_sv_post(int *flags, int k)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (threadIdx.x == 0) sv_store(flags, k);
+}
+
+__global__ void k_sv_wait(int *flags, int k, long long wait_ticks)
+{
+    if (threadIdx.x == 0) (void)sv_wait_ge(flags, 1, k, wait_ticks);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // ---------------------------------------------------------------------------
 // TRSM(m,k) for m = k+1 .. MT-1: C <- C L^{-T}, one workgroup (4 waves) per
 // 64-row strip of a tile, 32 columns at a time:
@@ -281,7 +372,7 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
 constexpr int TS_ROWS = 64;
 template <class T>
 __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N, int NB, int MT, int k,
-                                                  const T *__restrict__ Winv)
+                                                  const T *__restrict__ Winv, int m0)
 {
     __shared__ T P[32][TS_ROWS + 1];
     __shared__ T Ws[1024];
@@ -289,7 +380,7 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
     const int strips = (NB + TS_ROWS - 1) / TS_ROWS;
-    const int m = k + 1 + blockIdx.x / strips;
+    const int m = m0 + blockIdx.x / strips;
     const int r0 = (blockIdx.x % strips) * TS_ROWS;
     const int rows_m = tile_rows(N, NB, MT, m);
     if (r0 >= rows_m) return;
@@ -357,7 +448,7 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
 constexpr int TSK_ROWS = 16;
 template <class T>
 __global__ __launch_bounds__(256, 2) void k_trsm_tile_ks(T *__restrict__ Tt, int N, int NB, int MT, int k,
-                                                     const T *__restrict__ Winv)
+                                                     const T *__restrict__ Winv, int m0)
 {
     __shared__ T red[4][32][TSK_ROWS + 1];       // partial products [wave][col][row]
     __shared__ T P[32][TSK_ROWS + 1];
@@ -366,7 +457,7 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile_ks(T *__restrict__ Tt, int
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
     const int strips = (NB + TSK_ROWS - 1) / TSK_ROWS;
-    const int m = k + 1 + blockIdx.x / strips;
+    const int m = m0 + blockIdx.x / strips;
     const int r0 = (blockIdx.x % strips) * TSK_ROWS;
     const int rows_m = tile_rows(N, NB, MT, m);
     if (r0 >= rows_m) return;
@@ -577,7 +668,7 @@ __global__ __launch_bounds__(POTRF_THREADS) void k_potrf_diag(double *__restrict
 // owns 8 of the 32 columns of the current block.
 constexpr int TRSM_ROWS = 64;
 __global__ __launch_bounds__(256) void k_trsm_panel(double *__restrict__ T, int N, int NB, int MT, int k,
-                                                   const double *__restrict__ Winv)
+                                                   const double *__restrict__ Winv, int m0)
 {
     __shared__ double Xs[32][TRSM_ROWS + 1];   // X chunk, [p][row]
     __shared__ double Ls[32][33];              // L chunk, [p][col]
@@ -586,7 +677,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double *__restrict__ T, int 
     const int tid = threadIdx.x;
     const int row = tid & 63, g = tid >> 6;
     const int strips = (NB + TRSM_ROWS - 1) / TRSM_ROWS;
-    const int m = k + 1 + blockIdx.x / strips;
+    const int m = m0 + blockIdx.x / strips;
     const int strip = blockIdx.x % strips;
     const int rows_m = tile_rows(N, NB, MT, m);
     const int r0 = strip * TRSM_ROWS;
@@ -1122,6 +1213,22 @@ __global__ void k_fill_int(int *p, int v, int n)
 int g_panel_kernel = 1;
 // trailing-update engine: 0 = FP64 VALU, 1 = MFMA (default)
 int g_update_kernel = -1;
+// Diagonal factors through the persistent server (k_potrf_server): bit 0 for
+// FP64 factorizations, bit 1 for FP32 ones; -1 = not yet read from the
+// environment (DPLASMA_MI_POTRF_SERVER, default below).
+int g_potrf_server = -1;
+// bound on one wait of the server protocol, ticks of the 100 MHz counter (10 s)
+long long g_sv_wait_ticks = 1000000000LL;
+
+template <class T> bool use_server()
+{
+    if (g_potrf_server < 0) {
+        const char *e = getenv("DPLASMA_MI_POTRF_SERVER");
+        g_potrf_server = e ? atoi(e) & 3 : 3;
+    }
+    return (g_potrf_server >> (sizeof(T) == 8 ? 0 : 1)) & 1;
+}
+
 // FP32 trailing update of mi_spotrf_tiles and the FP32 step launches: 0 =
 // FP32 MFMA (default), 1 = BF16 MFMA, two-term split (the mixed solve passes
 // its choice as an argument instead)
@@ -1129,16 +1236,25 @@ int g_supdate = 0;
 
 // profiler names per precision
 template <class T> struct KN;
+// (update: launches on the main stream, which never overlap one another;
+// update_la: the look-ahead column updates on the panel stream)
 template <> struct KN<double> {
     static constexpr const char *potrf = "dpotrf_potrf_diag";
     static constexpr const char *trsm = "dpotrf_trsm_panel";
     static constexpr const char *update = "dpotrf_update";
+    static constexpr const char *update_la = "dpotrf_update_la";
+    static constexpr const char *potrf_server = "dpotrf_potrf_server";
+    static constexpr const char *potrf_wait = "dpotrf_potrf_wait";
 };
 template <> struct KN<float> {
     static constexpr const char *potrf = "spotrf_potrf_diag";
     static constexpr const char *trsm = "spotrf_trsm_panel";
     static constexpr const char *update = "spotrf_update";
+    static constexpr const char *update_la = "spotrf_update_la";
     static constexpr const char *update_bf16 = "spotrf_update_bf16x3";
+    static constexpr const char *update_bf16_la = "spotrf_update_bf16x3_la";
+    static constexpr const char *potrf_server = "spotrf_potrf_server";
+    static constexpr const char *potrf_wait = "spotrf_potrf_wait";
 };
 
 template <class T> bool valu_panel() { return std::is_same<T, double>::value && g_panel_kernel == 0; }
@@ -1154,8 +1270,10 @@ int potrf_lds_bytes(int NB, int *RC)
     return (PJB * (*RC) + 1024) * (int)sizeof(double);
 }
 
+// info_off: added to k*NB + j of a failing pivot (the offset of this tile
+// grid inside a larger matrix)
 template <class T>
-int launch_potrf(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, T *Winv)
+int launch_potrf(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, T *Winv, int info_off = 0)
 {
     int RC;
     int lds = potrf_lds_bytes<T>(NB, &RC);
@@ -1164,40 +1282,44 @@ int launch_potrf(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, T *
     const double fl = (double)n * n * n / 3.0;
     if (!valu_panel<T>()) {
         MI_LAUNCH(c, s, KN<T>::potrf, fl, 2.0 * n * n * sizeof(T),
-                  k_potrf_tile<T><<<1, 512, lds, s>>>(A, n, NB, RC, k * NB, c->d_info, Winv));
+                  k_potrf_tile<T><<<1, 512, lds, s>>>(A, n, NB, RC, info_off + k * NB, c->d_info, Winv));
     } else {
         MI_LAUNCH(c, s, KN<T>::potrf, fl, 2.0 * n * n * sizeof(T),
-                  k_potrf_diag<<<1, POTRF_THREADS, lds, s>>>((double *)A, n, NB, RC, k * NB, c->d_info,
-                                                              (double *)Winv));
+                  k_potrf_diag<<<1, POTRF_THREADS, lds, s>>>((double *)A, n, NB, RC, info_off + k * NB,
+                                                              c->d_info, (double *)Winv));
     }
     return 0;
 }
 
+// TRSM(m, k) for the ntile tiles m = m0 .. m0 + ntile - 1 (by default every
+// m > k).
 template <class T>
-int launch_trsm(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, const T *Winv)
+int launch_trsm(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, const T *Winv, int m0 = -1,
+                int ntile = -1)
 {
-    const int ntile = MT - k - 1;
+    if (m0 < 0) m0 = k + 1;
+    if (ntile < 0) ntile = MT - m0;
     if (ntile <= 0) return 0;
     double fl = 0.0;
-    for (int m = k + 1; m < MT; m++) fl += (double)tile_rows(N, NB, MT, m) * NB * NB;
+    for (int m = m0; m < m0 + ntile; m++) fl += (double)tile_rows(N, NB, MT, m) * NB * NB;
     if (!valu_panel<T>()) {
         // 64-row strips while there are many tiles; below ~50 tiles (FP64)
         // or ~40 (FP32) of NB = 512 the 16-row K-split kernel is faster alone
         const int strips = (NB + TS_ROWS - 1) / TS_ROWS;
         if (ntile * strips >= (sizeof(T) == 8 ? 400 : 320)) {
             MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
-                      k_trsm_tile<T><<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+                      k_trsm_tile<T><<<ntile * strips, 256, 0, s>>>(dT, N, NB, MT, k, Winv, m0));
         } else {
             // few tiles: 16-row strips with the K range split over the waves
             const int ks = (NB + TSK_ROWS - 1) / TSK_ROWS;
             MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
-                      k_trsm_tile_ks<T><<<ntile * ks, 256, 0, s>>>(dT, N, NB, MT, k, Winv));
+                      k_trsm_tile_ks<T><<<ntile * ks, 256, 0, s>>>(dT, N, NB, MT, k, Winv, m0));
         }
     } else {
         const int strips = (NB + TRSM_ROWS - 1) / TRSM_ROWS;
         MI_LAUNCH(c, s, KN<T>::trsm, fl, fl / NB * 2.0 * sizeof(T),
                   k_trsm_panel<<<ntile * strips, 256, 0, s>>>((double *)dT, N, NB, MT, k,
-                                                             (const double *)Winv));
+                                                             (const double *)Winv, m0));
     }
     return 0;
 }
@@ -1207,7 +1329,7 @@ int launch_trsm(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, cons
 // through the last column).
 template <class T>
 int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin,
-                      int m_off, int64_t ntiles, bool bf16_split)
+                      int m_off, int64_t ntiles, bool bf16_split, bool la = false)
 {
     if (n_begin >= MT || n_begin + m_off >= MT) return 0;
     int64_t avail = MT - n_begin - m_off;
@@ -1244,26 +1366,26 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
                   k_update_valu<<<(unsigned)grid, 256, 0, s>>>((double *)dT, N, NB, MT, k, n_begin, m_off,
                                                                nsub));
     else if (bf16)
-        MI_LAUNCH(c, s, KN<float>::update_bf16, fl, by,
+        MI_LAUNCH(c, s, la ? KN<float>::update_bf16_la : KN<float>::update_bf16, fl, by,
                   k_update_bf16x3<<<(unsigned)grid, 256, 0, s>>>((float *)dT, N, NB, MT, k, n_begin, m_off,
                                                                  nsub));
     else if (small)
-        MI_LAUNCH(c, s, KN<T>::update, fl, by,
+        MI_LAUNCH(c, s, la ? KN<T>::update_la : KN<T>::update, fl, by,
                   k_update_mfma<T, 64><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
     else
-        MI_LAUNCH(c, s, KN<T>::update, fl, by,
+        MI_LAUNCH(c, s, la ? KN<T>::update_la : KN<T>::update, fl, by,
                   k_update_mfma<T, BM><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
     return 0;
 }
 
 template <class T>
 int launch_update(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin, int n_end,
-                  bool bf16_split)
+                  bool bf16_split, bool la = false)
 {
     if (n_begin >= n_end) return 0;
     int64_t ntiles = 0;
     for (int n = n_begin; n < n_end; n++) ntiles += MT - n;
-    return launch_update_run<T>(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles, bf16_split);
+    return launch_update_run<T>(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles, bf16_split, la);
 }
 
 template <class T>
@@ -1281,15 +1403,20 @@ int prepare_potrf(int NB)
                   NB, lds, limit);
         return -4;
     }
-    if (lds > 65536)
+    if (lds > 65536) {
         MI_CHECK(hipFuncSetAttribute(!valu_panel<T>() ? (const void *)k_potrf_tile<T> : (const void *)k_potrf_diag,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        if (!valu_panel<T>())
+            MI_CHECK(hipFuncSetAttribute((const void *)k_potrf_server<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         lds));
+    }
     return 0;
 }
 
 // Elements of the diagonal-block inverses handed from POTRF(k) to TRSM(k),
 // two steps in flight.
 inline size_t potrf_ws_elems(int NB) { return (size_t)2 * ((NB + 31) / 32) * 1024; }
+
 
 // Whole factorization of the packed lower tiles dT (N, NB), resident, with a
 // two-step look-ahead over two streams (src/zpotrf_L.jdf's priorities give
@@ -1305,8 +1432,11 @@ inline size_t potrf_ws_elems(int NB) { return (size_t)2 * ((NB + 31) / 32) * 102
 // step k-1, not for all of it.
 // bf16_split: FP32 trailing updates on BF16 MFMA with the two-term split.
 // ws: potrf_ws_elems(NB) elements of workspace, or nullptr for the context's.
+// ring: cross-stream dependencies use the context's fixed ring of events
+// (created with the context) instead of its growing pool; every event is
+// waited on within three records of being recorded, far inside the ring.
 template <class T>
-int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws)
+int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws, bool ring = false)
 {
     *info = 0;
     if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
@@ -1317,45 +1447,100 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws)
     const size_t wsz = potrf_ws_elems(NB) / 2;      // elements per step
     if (!ws && !(ws = (T *)ws_get(c, WS_POTRF_WINV, potrf_ws_elems(NB) * sizeof(T)))) return -3;
     c->ev_next = 0;
+    size_t ring_next = 0;
+    auto ev = [&]() -> hipEvent_t { return ring ? c->ev_ring[ring_next++ % c->ev_ring.size()] : next_event(c); };
+    // the diagonal factors through the server (not for the allocation-free
+    // entry, whose flags would need uncached memory of their own)
+    const bool server = !ring && !valu_panel<T>() && use_server<T>() && MT > 1;
+    int *flags = nullptr;
+    if (server) {
+        if ((rc = ws_uncached(c, c->sv_flags, 4 * sizeof(int)))) return rc;
+        flags = (int *)c->sv_flags.p;
+    }
+    int RC = 0;
+    const int lds = potrf_lds_bytes<T>(NB, &RC);
+    // POTRF(k): a launch, or a post to the server and a wait for it
+    auto diag = [&](int k) -> int {
+        T *w = ws + (k & 1) * wsz;
+        if (!server) return launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k, w);
+        k_sv_post<<<1, 64, 0, c->s_panel>>>(flags, k);
+        MI_LAUNCH(c, c->s_panel, KN<T>::potrf_wait, 0.0, 0.0,
+                  k_sv_wait<<<1, 64, 0, c->s_panel>>>(flags, k, g_sv_wait_ticks));
+        return 0;
+    };
 
     MI_CHECK(hipEventRecord(c->t0, c->s_main));
     k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
-    hipEvent_t start = next_event(c);
+    if (server) k_sv_reset<<<1, 64, 0, c->s_main>>>(flags);
+    hipEvent_t start = ev();
     MI_CHECK(hipEventRecord(start, c->s_main));
     MI_CHECK(hipStreamWaitEvent(c->s_panel, start, 0));
+    hipEvent_t aux_done = nullptr;
+    // an error return after the server started stops it (abort word, then a
+    // wait for its stream), so that no server outlives its call
+    struct ServerGuard {
+        Ctx *c;
+        int *flags;
+        ~ServerGuard()
+        {
+            if (!flags) return;
+            const int one = 1;
+            (void)hipMemcpy(flags + 2, &one, sizeof(int), hipMemcpyHostToDevice);
+            (void)hipStreamSynchronize(c->s_aux);
+        }
+    } guard{c, nullptr};
+    if (server) {
+        MI_CHECK(hipStreamWaitEvent(c->s_aux, start, 0));
+        double fl = 0.0;
+        for (int k = 0; k < MT; k++) fl += std::pow((double)tile_rows(N, NB, MT, k), 3.0) / 3.0;
+        MI_LAUNCH(c, c->s_aux, KN<T>::potrf_server, fl, 0.0,
+                  k_potrf_server<T><<<1, 512, lds, c->s_aux>>>(dT, N, NB, MT, RC, 0, flags, ws, (int)wsz,
+                                                               c->d_info, g_sv_wait_ticks));
+        guard.flags = flags;
+        aux_done = ev();
+        MI_CHECK(hipEventRecord(aux_done, c->s_aux));
+    }
 
     // panel 0
-    if ((rc = launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, 0, ws))) return rc;
+    if ((rc = diag(0))) return rc;
     if ((rc = launch_trsm<T>(c, c->s_panel, dT, N, NB, MT, 0, ws))) return rc;
-    hipEvent_t pk = next_event(c);                 // TRSM(k) done
+    hipEvent_t pk = ev();                          // TRSM(k) done
     MI_CHECK(hipEventRecord(pk, c->s_panel));
     hipEvent_t col_ready = nullptr;                // column k+1 updated by step k-1
 
     for (int k = 0; k < MT - 1; k++) {
         // panel stream: column k+1 by step k, then the next panel
         if (col_ready) MI_CHECK(hipStreamWaitEvent(c->s_panel, col_ready, 0));
-        if ((rc = launch_update<T>(c, c->s_panel, dT, N, NB, MT, k, k + 1, k + 2, bf16_split))) return rc;
+        if ((rc = launch_update<T>(c, c->s_panel, dT, N, NB, MT, k, k + 1, k + 2, bf16_split, true))) return rc;
         T *w = ws + ((k + 1) & 1) * wsz;
-        if ((rc = launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
+        if ((rc = diag(k + 1))) return rc;
         if ((rc = launch_trsm<T>(c, c->s_panel, dT, N, NB, MT, k + 1, w))) return rc;
-        hipEvent_t pk_next = next_event(c);
+        hipEvent_t pk_next = ev();
         MI_CHECK(hipEventRecord(pk_next, c->s_panel));
         // main stream: the look-ahead column k+2, then the bulk of step k
         MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
         col_ready = nullptr;
         if (k + 2 < MT) {
             if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 2, k + 3, bf16_split))) return rc;
-            col_ready = next_event(c);
+            col_ready = ev();
             MI_CHECK(hipEventRecord(col_ready, c->s_main));
         }
         if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 3, MT, bf16_split))) return rc;
         pk = pk_next;
     }
     MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
+    if (aux_done) MI_CHECK(hipStreamWaitEvent(c->s_main, aux_done, 0));
     MI_CHECK(hipEventRecord(c->t1, c->s_main));
-    int h_info = 0;
+    int h_info = 0, h_flags[3] = {0, 0, 0};
     MI_CHECK(hipMemcpyAsync(&h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
+    if (server) MI_CHECK(hipMemcpyAsync(h_flags, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c->s_main));
     if ((rc = mi_synchronize(static_cast<mi_ctx_t *>(c)))) return rc;
+    guard.flags = nullptr;                        // the server has finished
+    if (server && (h_flags[2] || h_flags[1] != MT - 1)) {
+        set_error("tile Cholesky: the diagonal-factor server stopped at step %d of %d (wait limit)", h_flags[1],
+                  MT - 1);
+        return -5;
+    }
     float ms = 0.f;
     MI_CHECK(hipEventElapsedTime(&ms, c->t0, c->t1));
     c->last_ms = ms;
@@ -1407,6 +1592,26 @@ int step_update(mi_ctx_t *c, int N, int NB, T *dT, int k, int n_begin, int n_end
 }  // namespace
 
 namespace mi {
+void set_server_wait_ticks(long long ticks) { g_sv_wait_ticks = ticks; }
+
+// Single tile tasks on the packed lower image, enqueued on s (the bodies of
+// the four task classes of src/zpotrf_L.jdf:91-633, one tile each).
+int task_prepare_fp64(int NB) { return prepare_potrf<double>(NB); }
+int task_potrf_fp64(Ctx *c, hipStream_t s, double *dT, int N, int NB, int k, double *Winv)
+{
+    return launch_potrf<double>(c, s, dT, N, NB, (N + NB - 1) / NB, k, Winv);
+}
+int task_trsm_fp64(Ctx *c, hipStream_t s, double *dT, int N, int NB, int m, int k, const double *Winv)
+{
+    return launch_trsm<double>(c, s, dT, N, NB, (N + NB - 1) / NB, k, Winv, m, 1);
+}
+// tile (m, n), m >= n > k: A(m, n) -= A(m, k) A(n, k)^T (SYRK when m == n)
+int task_update_fp64(Ctx *c, hipStream_t s, double *dT, int N, int NB, int m, int n, int k)
+{
+    return launch_update_run<double>(c, s, dT, N, NB, (N + NB - 1) / NB, k, n, m - n, 1, false);
+}
+size_t task_winv_elems(int NB) { return potrf_ws_elems(NB) / 2; }
+
 int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split)
 {
     return potrf_tiles<float>(c, N, NB, dT, info, bf16_split, nullptr);
@@ -1426,6 +1631,13 @@ int mi_set_spotrf_update(int which)
 {
     if (which != 0 && which != 1) { set_error("FP32 update must be 0 (FP32 MFMA) or 1 (BF16 split)"); return -2; }
     g_supdate = which;
+    return 0;
+}
+
+int mi_set_potrf_server(int mask)
+{
+    if (mask < 0 || mask > 3) { set_error("server mask must be 0..3 (bit 0 FP64, bit 1 FP32)"); return -2; }
+    g_potrf_server = mask;
     return 0;
 }
 

@@ -1,6 +1,7 @@
 // mi_runtime.hip -- context, memory, copies and the event profiler of the
 // gfx950 kernel library (C ABI in include/dplasma_mi.h).
 #include <cstdarg>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include "mi_common.hpp"
@@ -24,7 +25,8 @@ void *ws_get(Ctx *c, int id, size_t bytes)
     if (b.p && b.bytes >= bytes) return b.p;
     if (b.p) {
         // the old buffer may still be read by work queued on this context
-        if (hipStreamSynchronize(c->s_panel) != hipSuccess || hipStreamSynchronize(c->s_main) != hipSuccess) {
+        if (hipStreamSynchronize(c->s_panel) != hipSuccess || hipStreamSynchronize(c->s_main) != hipSuccess ||
+            hipStreamSynchronize(c->s_aux) != hipSuccess) {
             set_error("workspace %d: stream synchronization failed", id);
             return nullptr;
         }
@@ -105,9 +107,14 @@ void prof_end(Ctx *c, hipStream_t s, ProfRec *r)
 // Resolve recorded launches into per-name totals (after a synchronize).
 static void prof_collect(Ctx *c)
 {
+    if (!c->recs.empty()) c->prof_calls++;
     for (auto &r : c->recs) {
-        float ms = 0.f;
+        float ms = 0.f, t_a = 0.f, t_b = 0.f;
         if (hipEventElapsedTime(&ms, r.ev0, r.ev1) != hipSuccess) ms = 0.f;
+        // launch interval relative to the call's start event (t0 is recorded
+        // first on the main stream by every blocking hot-path call)
+        if (hipEventElapsedTime(&t_a, c->t0, r.ev0) != hipSuccess) t_a = 0.f;
+        if (hipEventElapsedTime(&t_b, c->t0, r.ev1) != hipSuccess) t_b = t_a + ms;
         ProfAgg *a = nullptr;
         for (auto &x : c->agg)
             if (x.name == r.name) { a = &x; break; }
@@ -118,6 +125,9 @@ static void prof_collect(Ctx *c)
         }
         a->launches++;
         a->ms += ms;
+        a->t_start.push_back(t_a);
+        a->t_end.push_back(t_b);
+        a->call.push_back(c->prof_calls);
         a->flops += r.flops;
         a->bytes += r.bytes;
         c->prof_free.push_back(r.ev0);
@@ -156,11 +166,17 @@ int mi_ctx_create(int device, mi_ctx_t **out)
     // queue arbitration, not on stream priorities
     MI_CHECK(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
     MI_CHECK(hipStreamCreateWithFlags(&c->s_panel, hipStreamNonBlocking));
+    MI_CHECK(hipStreamCreateWithFlags(&c->s_aux, hipStreamNonBlocking));
     MI_CHECK(hipMalloc(&c->d_info, 64 * sizeof(int)));
     c->scratch_bytes = 1 << 20;
     MI_CHECK(hipMalloc(&c->d_scratch, c->scratch_bytes));
     MI_CHECK(hipEventCreate(&c->t0));
     MI_CHECK(hipEventCreate(&c->t1));
+    for (int i = 0; i < 16; i++) {
+        hipEvent_t e;
+        MI_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->ev_ring.push_back(e);
+    }
     *out = c;
     return 0;
 }
@@ -171,9 +187,11 @@ void mi_ctx_destroy(mi_ctx_t *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->s_main);
     (void)hipStreamSynchronize(c->s_panel);
+    (void)hipStreamSynchronize(c->s_aux);
     mi::prof_collect(c);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto e : c->prof_free) (void)hipEventDestroy(e);
+    for (auto e : c->ev_ring) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c->t0);
     (void)hipEventDestroy(c->t1);
     (void)hipFree(c->d_info);
@@ -184,15 +202,18 @@ void mi_ctx_destroy(mi_ctx_t *c)
         std::lock_guard<std::mutex> g(mi::g_retired_mu);
         if (c->wv_ticket.p) mi::g_retired.push_back(c->wv_ticket.p);
         if (c->wv_stage.p) mi::g_retired.push_back(c->wv_stage.p);
+        if (c->sv_flags.p) mi::g_retired.push_back(c->sv_flags.p);
     }
     (void)hipStreamDestroy(c->s_main);
     (void)hipStreamDestroy(c->s_panel);
+    (void)hipStreamDestroy(c->s_aux);
     delete c;
 }
 
 int mi_synchronize(mi_ctx_t *c)
 {
     MI_CHECK(hipStreamSynchronize(c->s_panel));
+    MI_CHECK(hipStreamSynchronize(c->s_aux));
     MI_CHECK(hipStreamSynchronize(c->s_main));
     c->ev_next = 0;
     if (c->prof) mi::prof_collect(c);
@@ -291,6 +312,44 @@ int mi_prof_reset(mi_ctx_t *c)
 {
     mi_synchronize(c);
     c->agg.clear();
+    c->prof_calls = 0;
+    return 0;
+}
+
+int mi_prof_busy_ms(mi_ctx_t *c, const char *kernel, double *busy_ms)
+{
+    mi::prof_collect(c);
+    *busy_ms = 0.0;
+    // union of the launch intervals of the kernel (all kernels when kernel
+    // is NULL), per blocking call (intervals of different calls share no
+    // time origin), summed over the calls
+    std::vector<std::pair<std::pair<int, double>, double>> iv;
+    // a kernel name ending in '*' selects every name with that prefix
+    const size_t kl = kernel ? strlen(kernel) : 0;
+    const bool prefix = kl > 0 && kernel[kl - 1] == '*';
+    for (auto &a : c->agg) {
+        if (kernel && !(prefix ? a.name.compare(0, kl - 1, kernel, kl - 1) == 0 : a.name == kernel)) continue;
+        for (size_t i = 0; i < a.t_start.size(); i++) iv.push_back({{a.call[i], a.t_start[i]}, a.t_end[i]});
+    }
+    if (iv.empty()) return kernel ? -1 : 0;
+    std::sort(iv.begin(), iv.end());
+    double tot = 0.0;
+    int cur_call = iv[0].first.first;
+    double s0 = iv[0].first.second, e0 = iv[0].second;
+    for (size_t i = 1; i < iv.size(); i++) {
+        const int call = iv[i].first.first;
+        const double s = iv[i].first.second, e = iv[i].second;
+        if (call == cur_call && s <= e0) {
+            if (e > e0) e0 = e;
+            continue;
+        }
+        tot += e0 - s0;
+        cur_call = call;
+        s0 = s;
+        e0 = e;
+    }
+    tot += e0 - s0;
+    *busy_ms = tot;
     return 0;
 }
 

@@ -15,8 +15,11 @@
 typedef enum {
     DPL_TP_POTRF = 1,
     DPL_TP_TRSM  = 2,
-    DPL_TP_PLGSY = 3
+    DPL_TP_PLGSY = 3,
+    DPL_TP_DTD   = 4
 } dpl_tp_kind_t;
+
+struct dpl_dtd_s;
 
 struct parsec_taskpool_s {
     dpl_tp_kind_t kind;
@@ -29,6 +32,7 @@ struct parsec_taskpool_s {
     int *info;
     int hmb;
     int rc;               /* runtime status of the last execution */
+    struct dpl_dtd_s *dtd;       /* DTD taskpools: the insertion state */
 };
 
 struct parsec_context_s {
@@ -87,5 +91,10 @@ int dpl_run_plgsy(parsec_context_t *ctx, double bump, int uplo, parsec_tiled_mat
                   unsigned long long seed);
 
 double dpl_now(void);
+
+/* DTD taskpools (dpl_dtd.c) */
+void dpl_dtd_attach(parsec_taskpool_t *tp, parsec_context_t *ctx);
+int  dpl_dtd_finish(parsec_taskpool_t *tp);
+void dpl_dtd_free(parsec_taskpool_t *tp);
 
 #endif

@@ -75,6 +75,7 @@ int parsec_fini(parsec_context_t **pctx)
 int parsec_context_add_taskpool(parsec_context_t *ctx, parsec_taskpool_t *tp)
 {
     if (!ctx || !tp) return -1;
+    if (tp->kind == DPL_TP_DTD) dpl_dtd_attach(tp, ctx);
     if (ctx->npending >= DPL_MAX_PENDING) return -1;
     ctx->pending[ctx->npending++] = tp;
     return 0;
@@ -97,6 +98,7 @@ int parsec_context_wait(parsec_context_t *ctx)
             tp->rc = dpl_run_trsm(ctx, tp->side, tp->uplo, tp->trans, tp->diag, tp->alpha, tp->Ac, tp->B);
             break;
         case DPL_TP_PLGSY: tp->rc = dpl_run_plgsy(ctx, tp->bump, tp->uplo, tp->A, tp->seed); break;
+        case DPL_TP_DTD: tp->rc = dpl_dtd_finish(tp); break;
         default: tp->rc = -1;
         }
         if (tp->rc && !status) status = tp->rc;
@@ -109,7 +111,12 @@ void parsec_devices_release_memory(void) { dpl_release_all(); }
 
 void parsec_devices_reset_load(parsec_context_t *ctx) { (void)ctx; }
 
-void parsec_taskpool_free(parsec_taskpool_t *tp) { free(tp); }
+void parsec_taskpool_free(parsec_taskpool_t *tp)
+{
+    if (!tp) return;
+    if (tp->kind == DPL_TP_DTD) dpl_dtd_free(tp);
+    free(tp);
+}
 
 void parsec_taskpool_sync_ids(void) {}
 

@@ -83,6 +83,23 @@ int dplasma_dposv_mixed(parsec_context_t *parsec, dplasma_enum_t uplo, parsec_ti
 int dplasma_dpotrf_adjoint(parsec_context_t *parsec, dplasma_enum_t uplo, const parsec_tiled_matrix_t *L,
                            parsec_tiled_matrix_t *Lbar);
 
+/* ---------------- DTD task classes (src/dtd_wrappers/dplasma_z_dtd.h:60-63;
+ * zherk -> dsyrk).  Arguments of each class, in insertion order:
+ *   dpotrf  uplo, m, A(k,k) [inout], lda, info [scratch: int *]
+ *   dtrsm   side, uplo, transA, diag, m, n, alpha, A [in], lda, B [inout], ldb
+ *   dsyrk   uplo, trans, n, k, alpha, A [in], lda, beta, C [inout], ldc
+ *   dgemm   transA, transB, m, n, k, alpha, A [in], lda, B [in], ldb, beta,
+ *           C [inout], ldc
+ * (values by address; alpha/beta are double).  Supported: the tasks of the
+ * tile Cholesky of a symmetric tile descriptor as
+ * tests/testing_zpotrf_dtd.c:124-302 inserts them for both uplo (alpha -1,
+ * beta 1 for the updates, alpha 1 for TRSM).  tile_full and devices are
+ * accepted and ignored: every task runs on the GPU. */
+parsec_task_class_t *parsec_dtd_create_dpotrf_task_class(parsec_taskpool_t *dtd_tp, int tile_full, int devices);
+parsec_task_class_t *parsec_dtd_create_dtrsm_task_class(parsec_taskpool_t *dtd_tp, int tile_full, int devices);
+parsec_task_class_t *parsec_dtd_create_dsyrk_task_class(parsec_taskpool_t *dtd_tp, int tile_full, int devices);
+parsec_task_class_t *parsec_dtd_create_dgemm_task_class(parsec_taskpool_t *dtd_tp, int tile_full, int devices);
+
 /* ---------------- triangular solve (dplasma_z.h:51,153,221) */
 int dplasma_dtrsm(parsec_context_t *parsec, dplasma_enum_t side, dplasma_enum_t uplo,
                   dplasma_enum_t trans, dplasma_enum_t diag, double alpha,

@@ -126,10 +126,11 @@ int mi_dpotrs_tiles(mi_ctx_t *ctx, int N, int NB, const double *dT, int NRHS, do
  * through uncached device memory) whenever NB % 32 == 0; 0 = one
  * diagonal-solve and one update launch per tile.  Process-wide. */
 int mi_set_solve_engine(int which);
-/* Bound on one hand-off wait of the wavefront solve, in seconds (default
- * 10; 0 gives up at once).  A wait that exceeds it ends the launch early:
- * the blocking call then returns -5 and its right-hand sides are not valid.
- * Process-wide. */
+/* Bound on one hand-off wait between workgroups -- the wavefront solve's
+ * and the diagonal-factor server's (mi_set_potrf_server) -- in seconds
+ * (default 10; 0 gives up at once).  A wait that exceeds it ends the
+ * launches early: the blocking call then returns -5 and its outputs are not
+ * valid.  Process-wide. */
 int mi_set_solve_wait_limit(double seconds);
 /* FP32 tiles and right-hand sides: the "s" precision of the two calls above. */
 int mi_strsm_LL(mi_ctx_t *ctx, int trans, int diag, float alpha, int N, int NB,
@@ -171,6 +172,14 @@ int mi_dpotrf_step_potrf(mi_ctx_t *ctx, int N, int NB, double *dT, int k, int *i
  * MFMA 16x16x4 (default; two resident waves per SIMD keep it at full rate).
  * Process-wide. */
 int mi_set_update_engine(int which);
+/* Diagonal factors of mi_dpotrf_tiles / mi_spotrf_tiles through one
+ * persistent workgroup that holds a CU for the whole factorization and runs
+ * POTRF(k) as the panel stream posts it (bit 0: FP64, bit 1: FP32), instead
+ * of one launch per step, which has to wait for a CU to empty beside the
+ * trailing update.  Its waits are bounded like the solve's (a factorization
+ * whose server stops returns -5).  Process-wide; default 3, or the
+ * environment variable DPLASMA_MI_POTRF_SERVER. */
+int mi_set_potrf_server(int mask);
 /* Engine of POTRF(k) and the panel TRSM: 0 = VALU kernels, 1 = MFMA panel
  * updates with register/LDS diagonal blocks (default).  Process-wide. */
 int mi_set_panel_engine(int which);
@@ -185,6 +194,29 @@ int mi_spotrf_step_update(mi_ctx_t *ctx, int N, int NB, float *dT, int k, int n_
  * MFMA with FP32 accumulation on a two-term bf16 split of both operands
  * (three products, ~16 significant bits).  Process-wide. */
 int mi_set_spotrf_update(int which);
+
+/* ------------------------------------------------------------ task insertion */
+/* Task-insertion engine behind the DTD front-end of dplasma.h (the tile
+ * tasks of src/dtd_wrappers/z{potrf,trsm,herk,gemm}.c as
+ * tests/testing_zpotrf_dtd.c:91-302 inserts them).  Tasks work on a packed
+ * lower FP64 tile image dT (N, NB) in device memory; each is enqueued when
+ * inserted and waits only for the earlier tasks whose tiles it reads or
+ * writes (read-after-write, write-after-write, write-after-read), so any
+ * insertion order that is valid sequentially gives the sequential result.
+ *   POTRF(k)      A(k,k) = chol(A(k,k)); info as mi_dpotrf_tiles
+ *   TRSM(m,k)     A(m,k) <- A(m,k) A(k,k)^-T                 (m > k)
+ *   SYRK(k,m)     A(m,m) <- A(m,m) - A(m,k) A(m,k)^T          (m > k)
+ *   GEMM(m,n,k)   A(m,n) <- A(m,n) - A(m,k) A(n,k)^T          (m > n > k)
+ * mi_dtd_wait blocks until every inserted task has run and returns the
+ * info word and the number of tasks; the engine can take further tasks. */
+typedef struct mi_dtd_s mi_dtd_t;
+int  mi_dtd_create(mi_ctx_t *ctx, int N, int NB, double *dT, mi_dtd_t **dtd);
+int  mi_dtd_potrf(mi_dtd_t *dtd, int k);
+int  mi_dtd_trsm(mi_dtd_t *dtd, int m, int k);
+int  mi_dtd_syrk(mi_dtd_t *dtd, int k, int m);
+int  mi_dtd_gemm(mi_dtd_t *dtd, int m, int n, int k);
+int  mi_dtd_wait(mi_dtd_t *dtd, int *info, long *ntasks);
+void mi_dtd_destroy(mi_dtd_t *dtd);
 
 /* ------------------------------------------------------------ mixed precision */
 /* Mixed-precision solve of A X = B (no counterpart in the reference: the
@@ -289,6 +321,12 @@ int mi_prof_enable(mi_ctx_t *ctx, int on);
 int mi_prof_reset(mi_ctx_t *ctx);
 int mi_prof_get(mi_ctx_t *ctx, const char *kernel, int *launches, double *total_ms,
                 double *flops, double *bytes);
+/* Time during which at least one launch of `kernel` (of any recorded
+ * kernel when NULL) was running, ms: the union of the launch intervals
+ * within each blocking call, summed over the calls -- launches overlapping
+ * on different streams count once.  A name ending in '*' selects every
+ * kernel name with that prefix. */
+int mi_prof_busy_ms(mi_ctx_t *ctx, const char *kernel, double *busy_ms);
 /* Number of distinct kernel names recorded, and the i-th name. */
 int mi_prof_count(mi_ctx_t *ctx);
 const char *mi_prof_name(mi_ctx_t *ctx, int i);

@@ -127,6 +127,60 @@ void *parsec_data_allocate(size_t size);
 void  parsec_data_free(void *p);
 size_t parsec_datadist_getsizeoftype(parsec_matrix_type_t type);
 
+/* ------------------------------------------------------- task insertion (DTD) */
+/* The insert-task interface of PaRSEC (interfaces/dtd/insert_function.h) as
+ * tests/testing_zpotrf_dtd.c uses it, over the tile tasks of this library
+ * (the task classes come from dplasma.h, parsec_dtd_create_d*_task_class).
+ * A DTD taskpool binds to one symmetric tile descriptor at its first task:
+ * the uplo part is staged into HBM once, every inserted task is enqueued on
+ * the GPU at insertion (ordered only by the tiles it reads and writes), and
+ * parsec_dtd_data_flush_all / parsec_taskpool_wait / parsec_context_wait
+ * wait for the tasks and write the tiles back to host memory.
+ * parsec_dtd_insert_task_with_task_class takes (flag, pointer) pairs in the
+ * order of the task class's arguments, ended by PARSEC_DTD_ARG_END: values
+ * by address, tiles from PARSEC_DTD_TILE_OF.  Flags are accepted and not
+ * interpreted (the class fixes each argument's access mode).  Returns 0, or
+ * a negative code for a task outside the tile-Cholesky pattern (printed on
+ * stderr; nothing is enqueued). */
+typedef struct parsec_task_class_s parsec_task_class_t;
+typedef struct parsec_dtd_tile_s parsec_dtd_tile_t;
+typedef uint64_t parsec_data_key_t;
+
+#define PARSEC_DTD_ARG_END    0
+#define PARSEC_DTD_EMPTY_FLAG 0
+#define PARSEC_VALUE          0x00010000
+#define PARSEC_SCRATCH        0x00020000
+#define PASSED_BY_REF         (-1)
+#define PARSEC_INPUT          0x00100000
+#define PARSEC_OUTPUT         0x00200000
+#define PARSEC_INOUT          (PARSEC_INPUT | PARSEC_OUTPUT)
+#define PARSEC_AFFINITY       0x01000000
+#define PARSEC_PUSHOUT        0x02000000
+#define PARSEC_DEV_CPU        0x01
+#define PARSEC_DEV_CUDA       0x02
+#define PARSEC_DEV_HIP        0x04
+#define PARSEC_DEV_ALL        0x7f
+
+parsec_taskpool_t *parsec_dtd_taskpool_new(void);
+int  parsec_dtd_data_collection_init(parsec_data_collection_t *dc);
+void parsec_dtd_data_collection_fini(parsec_data_collection_t *dc);
+/* key of tile (m, n) of a shim descriptor: m + n * lmt */
+parsec_data_key_t parsec_shim_data_key(const parsec_tiled_matrix_t *A, int m, int n);
+parsec_dtd_tile_t *parsec_dtd_tile_of(parsec_data_collection_t *dc, parsec_data_key_t key);
+/* as in PaRSEC, DC names a descriptor variable dc for which a pointer
+ * __dc<DC> is in scope */
+#define PARSEC_DTD_TILE_OF(DC, I, J)                                                        \
+    parsec_dtd_tile_of((parsec_data_collection_t *)(__dc##DC),                              \
+                       parsec_shim_data_key((const parsec_tiled_matrix_t *)(__dc##DC), (I), (J)))
+int  parsec_dtd_insert_task_with_task_class(parsec_taskpool_t *tp, parsec_task_class_t *tc, int priority,
+                                            int device_type, ...);
+void parsec_dtd_data_flush(parsec_taskpool_t *tp, parsec_dtd_tile_t *tile);
+void parsec_dtd_data_flush_all(parsec_taskpool_t *tp, parsec_data_collection_t *dc);
+int  parsec_taskpool_wait(parsec_taskpool_t *tp);
+void parsec_dtd_task_class_release(parsec_taskpool_t *tp, parsec_task_class_t *tc);
+/* tasks inserted into a DTD taskpool so far */
+long parsec_shim_dtd_ntasks(const parsec_taskpool_t *tp);
+
 /* Address of tile (m,n) in the descriptor's host storage, or NULL when the
  * descriptor does not store it (upper tiles of a lower symmetric one). */
 void *parsec_shim_tile_addr(const parsec_tiled_matrix_t *A, int m, int n);
