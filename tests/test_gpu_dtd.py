@@ -120,7 +120,7 @@ def test_dtd_insert_rejects_outside_pattern(ctx, dp, po):
                                                   ctypes.c_int(0x7f), *args)
     assert rc == -2
     args[11] = Dv(-1.0)
-    args[14] = tile(1, 0)                                 # A's row is not C's row
+    args[13] = tile(1, 0)                                 # A's row is not C's row
     rc = D.parsec_dtd_insert_task_with_task_class(ctypes.c_void_p(tp), ctypes.c_void_p(gemm), ctypes.c_int(1),
                                                   ctypes.c_int(0x7f), *args)
     assert rc == -2
