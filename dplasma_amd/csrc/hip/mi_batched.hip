@@ -587,6 +587,7 @@ int mi_batch_create(mi_ctx_t *c, int count, const int *n, mi_batch_t **plan)
     if (e == hipSuccess && count) e = hipMemcpy(p->d_aoff, ao.data(), count * sizeof(int64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess && count) e = hipMemcpy(p->d_boff, bo.data(), count * sizeof(int64_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
+        (void)hipGetLastError();
         set_error("mi_batch_create: %s", hipGetErrorString(e));
         mi_batch_destroy(p);
         return -1000 - (int)e;

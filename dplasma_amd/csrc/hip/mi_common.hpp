@@ -182,10 +182,14 @@ int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS
 // the opaque handle of the C ABI is the context itself
 struct mi_ctx_s : public mi::Ctx {};
 
+// A failed call also resets HIP's per-thread last error, so that a
+// recoverable failure (out of memory) is not reported again by the next
+// launch's check; a sticky device error is reported again anyway.
 #define MI_CHECK(call)                                                            \
     do {                                                                          \
         hipError_t e_ = (call);                                                   \
         if (e_ != hipSuccess) {                                                   \
+            (void)hipGetLastError();                                              \
             mi::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,              \
                           hipGetErrorString(e_));                                 \
             return -1000 - (int)e_;                                               \
@@ -193,10 +197,13 @@ struct mi_ctx_s : public mi::Ctx {};
     } while (0)
 
 // Launch with optional event bracketing for the profiler.
+// The last error is cleared right before the launch, so that the check
+// after it reports this launch and not an earlier ignored failure.
 #define MI_LAUNCH(ctx, stream, name, flops, bytes, ...)                           \
     do {                                                                          \
         mi::ProfRec r_{};                                                         \
         if ((ctx)->prof) mi::prof_begin((ctx), (stream), (name), (flops), (bytes), &r_); \
+        (void)hipGetLastError();                                                  \
         __VA_ARGS__;                                                              \
         hipError_t e_ = hipGetLastError();                                        \
         if (e_ != hipSuccess) {                                                   \

@@ -187,6 +187,7 @@ int mi_dspd_from_orth_tiles(mi_ctx_t *c, int N, int NB, const double *dQ, double
     const double one = 1.0, zero = 0.0;
     MI_CHECK(hipMalloc(&W, (size_t)N * N * sizeof(double)));
     if (hipMalloc(&C, (size_t)N * N * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
         (void)hipFree(W);
         set_error("out of device memory for the generator");
         return -3;
@@ -225,6 +226,7 @@ int mi_dbatch_spd(mi_ctx_t *c, int count, const int *n, uint64_t seed, double *d
     }
     MI_CHECK(hipMalloc(&G, (size_t)nmax * nmax * sizeof(double)));
     if (hipMalloc(&dn, count * sizeof(int)) != hipSuccess || hipMalloc(&doff, count * sizeof(int64_t)) != hipSuccess) {
+        (void)hipGetLastError();
         set_error("out of device memory for the batch generator");
         rc = -3;
         goto out;

@@ -1215,7 +1215,11 @@ int g_panel_kernel = 1;
 int g_update_kernel = -1;
 // Diagonal factors through the persistent server (k_potrf_server): bit 0 for
 // FP64 factorizations, bit 1 for FP32 ones; -1 = not yet read from the
-// environment (DPLASMA_MI_POTRF_SERVER, default below).
+// environment (DPLASMA_MI_POTRF_SERVER, default 0: off).  The server waits
+// on kernels of another stream; HIP maps streams onto a few hardware queues,
+// and a kernel queued behind the server in its queue cannot start until the
+// server ends, so with several contexts (or other streams) the server and
+// its feeder can wait on each other until the bounded wait gives up.
 int g_potrf_server = -1;
 // bound on one wait of the server protocol, ticks of the 100 MHz counter (10 s)
 long long g_sv_wait_ticks = 1000000000LL;
@@ -1224,7 +1228,7 @@ template <class T> bool use_server()
 {
     if (g_potrf_server < 0) {
         const char *e = getenv("DPLASMA_MI_POTRF_SERVER");
-        g_potrf_server = e ? atoi(e) & 3 : 3;
+        g_potrf_server = e ? atoi(e) & 3 : 0;
     }
     return (g_potrf_server >> (sizeof(T) == 8 ? 0 : 1)) & 1;
 }

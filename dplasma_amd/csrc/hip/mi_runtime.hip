@@ -112,9 +112,12 @@ static void prof_collect(Ctx *c)
         float ms = 0.f, t_a = 0.f, t_b = 0.f;
         if (hipEventElapsedTime(&ms, r.ev0, r.ev1) != hipSuccess) ms = 0.f;
         // launch interval relative to the call's start event (t0 is recorded
-        // first on the main stream by every blocking hot-path call)
+        // first on the main stream by every blocking hot-path call; the
+        // single-step entries do not record it, and then the interval is
+        // the launch's own)
         if (hipEventElapsedTime(&t_a, c->t0, r.ev0) != hipSuccess) t_a = 0.f;
         if (hipEventElapsedTime(&t_b, c->t0, r.ev1) != hipSuccess) t_b = t_a + ms;
+        (void)hipGetLastError();                   // failures above are handled
         ProfAgg *a = nullptr;
         for (auto &x : c->agg)
             if (x.name == r.name) { a = &x; break; }

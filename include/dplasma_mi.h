@@ -177,8 +177,11 @@ int mi_set_update_engine(int which);
  * POTRF(k) as the panel stream posts it (bit 0: FP64, bit 1: FP32), instead
  * of one launch per step, which has to wait for a CU to empty beside the
  * trailing update.  Its waits are bounded like the solve's (a factorization
- * whose server stops returns -5).  Process-wide; default 3, or the
- * environment variable DPLASMA_MI_POTRF_SERVER. */
+ * whose server stops returns -5).  Off by default: HIP maps streams onto a
+ * few hardware queues, so with more than one context (or other busy
+ * streams) a kernel the server waits for can be queued behind it, and the
+ * factorization then ends with -5 after the wait limit.  Process-wide;
+ * default 0, or the environment variable DPLASMA_MI_POTRF_SERVER. */
 int mi_set_potrf_server(int mask);
 /* Engine of POTRF(k) and the panel TRSM: 0 = VALU kernels, 1 = MFMA panel
  * updates with register/LDS diagonal blocks (default).  Process-wide. */

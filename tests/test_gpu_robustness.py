@@ -169,3 +169,30 @@ def test_solve_wait_limit_gives_up_and_recovers(dev, po, dp):
     assert mi.mi_set_solve_wait_limit(-1.0) == -2
     dev.free(dT)
     dev.free(dB)
+
+
+def test_dpotrf_image_larger_than_hbm_returns_negative(ctx, dp):
+    """A symmetric descriptor whose packed tile image (N = 300000, NB = 512:
+    ~361 GB) exceeds the 288 GB of HBM: dplasma_dpotrf returns a negative
+    code (out of device memory) instead of aborting, and the context stays
+    usable.  The device allocation fails before any host tile is read, so
+    the descriptor's host storage is a one-element stand-in."""
+    _, D = dp.libs()
+    N, NB = 300000, 512
+    desc = dp.SymBlockCyclic()
+    D.parsec_matrix_sym_block_cyclic_init(ctypes.byref(desc), dp.MATRIX_DOUBLE, 0, NB, NB, N, N,
+                                          0, 0, N, N, 1, 1, dp.LOWER)
+    host = np.zeros(1)
+    desc.mat = host.ctypes.data
+    mt = (N + NB - 1) // NB
+    assert mt * (mt + 1) // 2 * NB * NB * 8 > 300e9
+    print("out-of-HBM dpotrf: N=300000 NB=512", flush=True)
+    rc = D.dplasma_dpotrf(ctx.ptr, dp.LOWER, ctypes.byref(desc))
+    assert rc < 0
+    assert "memory" in dp.last_error().lower()
+    D.parsec_tiled_matrix_destroy(ctypes.byref(desc))
+    # the context still factors a small matrix
+    A = dp.SymMatrix(ctx, 378, 93, dp.LOWER)
+    assert ctx.dplgsy(378.0, dp.LOWER, A, 3872) == 0
+    assert ctx.dpotrf(dp.LOWER, A) == 0
+    A.free()

@@ -29,7 +29,7 @@ def factor(dev, T, N, NB, server, fp32=False, bf16=False):
             dev.check(M.mi_dpotrf_tiles(dev.h, N, NB, d, ctypes.byref(info)), "dpotrf")
             out = dev.download(d, T.shape)
     finally:
-        M.mi_set_potrf_server(3)
+        M.mi_set_potrf_server(0)
     dev.free(d)
     return out, info.value
 
@@ -81,6 +81,7 @@ def test_server_wait_limit(dev, po, dp):
         msg = dp.last_error()
     finally:
         assert mi.mi_set_solve_wait_limit(10.0) == 0
+        assert mi.mi_set_potrf_server(0) == 0
     assert rc == -5, (rc, msg)
     assert "server" in msg
     dev.free(d)
