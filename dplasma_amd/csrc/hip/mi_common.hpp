@@ -161,6 +161,9 @@ int trsm_status(Ctx *c);
 // FP32 tile factorization (mi_spotrf_tiles) with the trailing-update engine
 // given per call: bf16_split = BF16 MFMA on the two-term split.
 int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split);
+// Reads, once per device and with that device current, the device address of
+// the fixed scratch of mi_dpotrf_tiles_noalloc (mi_potrf.hip).
+void noalloc_setup(int device);
 // bound on the waits of the diagonal-factor server (mi_set_solve_wait_limit)
 void set_server_wait_ticks(long long ticks);
 // Single-tile tasks of the tile Cholesky on the packed lower FP64 image,

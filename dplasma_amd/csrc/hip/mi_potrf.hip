@@ -1419,7 +1419,7 @@ int prepare_potrf(int NB)
 
 // Elements of the diagonal-block inverses handed from POTRF(k) to TRSM(k),
 // two steps in flight.
-inline size_t potrf_ws_elems(int NB) { return (size_t)2 * ((NB + 31) / 32) * 1024; }
+constexpr size_t potrf_ws_elems(int NB) { return (size_t)2 * ((NB + 31) / 32) * 1024; }
 
 
 // Whole factorization of the packed lower tiles dT (N, NB), resident, with a
@@ -1622,6 +1622,43 @@ int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_sp
 }
 }  // namespace mi
 
+// ---------------------------------------------------------------------------
+// Allocation-free FP64 factorization (mi_dpotrf_tiles_noalloc).  Its only
+// scratch memory, the 32 x 32 diagonal-block inverses that POTRF(k) hands to
+// TRSM(k) for two steps in flight, lives in this file-scope device array
+// instead of a context workspace: one copy per GPU, held until the process
+// ends, shared by every call on that GPU.  POTRF(k) writes each 1024-element
+// block whole before TRSM(k) reads it, so nothing has to be cleared between
+// calls.  On the host the array's name is only a stand-in: host code uses
+// the device address that noalloc_setup reads once per device.
+namespace {
+constexpr int NOALLOC_MAX_NB = 1024;
+constexpr int NOALLOC_MAX_DEV = 64;
+__device__ __align__(256) double scratch[potrf_ws_elems(NOALLOC_MAX_NB)];   // 512 KiB
+
+std::mutex g_noalloc_mu;
+bool g_noalloc_done[NOALLOC_MAX_DEV];
+hipError_t g_noalloc_err[NOALLOC_MAX_DEV];
+double *g_noalloc_ptr[NOALLOC_MAX_DEV];
+}  // namespace
+
+namespace mi {
+// Called by mi_ctx_create with `device` current: the device address of
+// `scratch` on that GPU, read once per device and kept.
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEV) return;
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (g_noalloc_done[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) (void)hipGetLastError();
+    g_noalloc_err[device] = e;
+    g_noalloc_ptr[device] = e == hipSuccess ? static_cast<double *>(p) : nullptr;
+    g_noalloc_done[device] = true;
+}
+}  // namespace mi
+
 extern "C" {
 
 int mi_set_panel_engine(int which)
@@ -1685,6 +1722,39 @@ int mi_spotrf_step_update(mi_ctx_t *c, int N, int NB, float *dT, int k, int n_be
 int mi_dpotrf_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
 {
     return potrf_tiles<double>(c, N, NB, dT, info, false, nullptr);
+}
+
+int mi_dpotrf_tiles_noalloc(mi_ctx_t *c, int N, int NB, double *dT, int *info)
+{
+    *info = 0;
+    if (N < 0 || NB <= 0) { set_error("mi_dpotrf_tiles_noalloc: bad sizes N=%d NB=%d", N, NB); return -2; }
+    if (NB > NOALLOC_MAX_NB) {
+        set_error("mi_dpotrf_tiles_noalloc: NB=%d exceeds the limit of its fixed scratch (NB <= %d)", NB,
+                  NOALLOC_MAX_NB);
+        return -2;
+    }
+    double *ptr = nullptr;
+    hipError_t e = hipErrorInvalidDevice;
+    if (c->device >= 0 && c->device < NOALLOC_MAX_DEV) {
+        std::lock_guard<std::mutex> g(g_noalloc_mu);
+        if (g_noalloc_done[c->device]) {
+            ptr = g_noalloc_ptr[c->device];
+            e = g_noalloc_err[c->device];
+        }
+    }
+    if (!ptr) {
+        set_error("mi_dpotrf_tiles_noalloc: no scratch address for device %d (%s)", c->device,
+                  hipGetErrorString(e));
+        return -1;
+    }
+    // per-step POTRF launches and the context's fixed event ring (the
+    // diagonal-factor server and the event pool would allocate); the
+    // profiler creates events on demand, so it is off for this call
+    const bool prof = c->prof;
+    c->prof = false;
+    const int rc = potrf_tiles<double>(c, N, NB, dT, info, false, ptr, true);
+    c->prof = prof;
+    return rc;
 }
 
 int mi_spotrf_tiles(mi_ctx_t *c, int N, int NB, float *dT, int *info)

@@ -180,6 +180,7 @@ int mi_ctx_create(int device, mi_ctx_t **out)
         MI_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         c->ev_ring.push_back(e);
     }
+    mi::noalloc_setup(device);
     *out = c;
     return 0;
 }

@@ -114,6 +114,16 @@ int mi_dpotrf_tiles(mi_ctx_t *ctx, int N, int NB, double *dT, int *info);
  * (v_mfma_f32_16x16x4_f32) trailing update, FP32 diagonal factor and panel
  * solve, same info rule.  Blocking. */
 int mi_spotrf_tiles(mi_ctx_t *ctx, int N, int NB, float *dT, int *info);
+/* mi_dpotrf_tiles without any device allocation, for callers that cannot
+ * allocate (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, results and
+ * info as mi_dpotrf_tiles (bitwise the same factor); POTRF(k) runs as one
+ * launch per step.  Its scratch (the diagonal-block inverses of two steps in
+ * flight) is a fixed 512 KiB device array, one per GPU, so NB <= 1024
+ * (larger NB: -2 and an error message).  Calls that overlap on one GPU race
+ * on that array, whichever context they use: serialize them.  The context's
+ * profiler is not used by this call.  Blocking. */
+int mi_dpotrf_tiles_noalloc(mi_ctx_t *ctx, int N, int NB, double *dT, int *info);
 /* Left/Lower tile triangular solve, B <- alpha op(L)^{-1} B, op = NoTrans
  * (111) or Trans (112), diag NonUnit (131) / Unit (132):
  * src/ztrsm_LLN.jdf, src/ztrsm_LLT.jdf.  Blocking. */
