@@ -167,7 +167,6 @@ def libs():
     _sig(M, "mi_dpotrf_step_update", i, [vp, i, i, vp, i, i, i])
     _sig(M, "mi_set_update_engine", i, [i])
     _sig(M, "mi_set_panel_engine", i, [i])
-    _sig(M, "mi_set_potrf_server", i, [i])
     _sig(M, "mi_set_spotrf_update", i, [i])
     _sig(M, "mi_set_solve_engine", i, [i])
     _sig(M, "mi_set_solve_wait_limit", i, [d])

@@ -111,7 +111,6 @@ struct Ctx {
     int device = 0;
     hipStream_t s_main = nullptr;   // trailing updates, solves, copies
     hipStream_t s_panel = nullptr;  // diagonal factorization + panel solve (look-ahead)
-    hipStream_t s_aux = nullptr;    // the persistent diagonal-factor server
     std::vector<hipEvent_t> ev_pool;   // cross-stream dependencies
     size_t ev_next = 0;
     std::vector<hipEvent_t> ev_ring;   // fixed ring (created with the context)
@@ -122,7 +121,6 @@ struct Ctx {
     // hand-off memory of the wavefront solve (uncached; an uncached buffer
     // that a kernel has read is never freed, see ws_uncached)
     DevBuf wv_ticket, wv_stage;
-    DevBuf sv_flags;                // flags of the diagonal-factor server (uncached)
     bool prof = false;
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> prof_free;
@@ -164,8 +162,6 @@ int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_sp
 // Reads, once per device and with that device current, the device address of
 // the fixed scratch of mi_dpotrf_tiles_noalloc (mi_potrf.hip).
 void noalloc_setup(int device);
-// bound on the waits of the diagonal-factor server (mi_set_solve_wait_limit)
-void set_server_wait_ticks(long long ticks);
 // Single-tile tasks of the tile Cholesky on the packed lower FP64 image,
 // enqueued on s (mi_potrf.hip): POTRF(k) writes the inverses of its 32 x 32
 // diagonal blocks to Winv (task_winv_elems(NB) elements), which TRSM(m, k)

@@ -282,89 +282,6 @@ __global__ __launch_bounds__(512) void k_potrf_tile(T *__restrict__ A, int n, in
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal-factor server.  POTRF(k) needs a whole CU (512 threads, most of
-// the LDS), and inside the factorization a CU only empties once the bulk
-// update of the step has dispatched all its workgroups, so a POTRF launch
-// waits there.  Instead, one persistent workgroup with the same shape holds
-// a CU for the whole factorization and runs POTRF(k0), POTRF(k0+1), ... as
-// the panel stream posts them:
-//   panel stream  ... column k updated -> k_sv_post(k) -> k_sv_wait(k) -> TRSM(k)
-//   server        wait posted >= k; acquire; POTRF(k); release; done = k
-// flags (uncached device memory): [0] last posted step, [1] last finished
-// step, [2] abort.  Every wait is bounded by wait_ticks of the 100 MHz
-// real-time counter: a wait that runs out sets the abort word, and every
-// wait ends once it is set, so all launches drain and the host reports it.
-// Tile data is ordinary device memory, ordered by agent-scope release
-// (server after POTRF; the posting kernel runs after the column update has
-// completed) and acquire fences (server before POTRF, k_sv_wait before TRSM).
-__device__ inline int sv_load(const int *p)
-{
-    asm volatile("" : "+v"(p));
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void sv_store(int *p, int v)
-{
-    asm volatile("" : "+v"(p));
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Waits (one thread) until flags[idx] >= k; false when the abort word is
-// set or the budget ran out (then it sets the abort word).
-__device__ inline bool sv_wait_ge(int *flags, int idx, int k, long long wait_ticks)
-{
-    const long long t_end = (long long)__builtin_amdgcn_s_memrealtime() + wait_ticks;
-    if (wait_ticks == 0) {                    // a zero budget ends every wait at once
-        sv_store(flags + 2, 1);
-        return false;
-    }
-    for (;;) {
-        if (sv_load(flags + 2)) return false;
-        if (sv_load(flags + idx) >= k) return true;
-        if ((long long)__builtin_amdgcn_s_memrealtime() >= t_end) {
-            sv_store(flags + 2, 1);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(512) void k_potrf_server(T *__restrict__ dT, int N, int NB, int MT, int RC, int k0,
-                                                     int *__restrict__ flags, T *__restrict__ ws, int wsz,
-                                                     int *__restrict__ d_info, long long wait_ticks)
-{
-    __shared__ int s_go;
-    for (int k = k0; k < MT; k++) {
-        if (threadIdx.x == 0) s_go = sv_wait_ge(flags, 0, k, wait_ticks) ? 1 : 0;
-        __syncthreads();
-        if (!s_go) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        T *A = dT + tile_index(MT, k, k) * (int64_t)NB * NB;
-        potrf_tile_dev<T>(A, tile_rows(N, NB, MT, k), NB, RC, k * NB, d_info, ws + (k & 1) * (int64_t)wsz);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (threadIdx.x == 0) sv_store(flags + 1, k);
-    }
-}
-
-__global__ void k_sv_reset(int *flags)
-{
-    if (threadIdx.x < 3) sv_store(flags + threadIdx.x, threadIdx.x < 2 ? -1 : 0);
-}
-
-__global__ void k_sv_post(int *flags, int k)
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (threadIdx.x == 0) sv_store(flags, k);
-}
-
-__global__ void k_sv_wait(int *flags, int k, long long wait_ticks)
-{
-    if (threadIdx.x == 0) (void)sv_wait_ge(flags, 1, k, wait_ticks);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
-
-// ---------------------------------------------------------------------------
 // TRSM(m,k) for m = k+1 .. MT-1: C <- C L^{-T}, one workgroup (4 waves) per
 // 64-row strip of a tile, 32 columns at a time:
 //   P = C[:, j0:j0+32] - X[:, 0:j0] L[j0:j0+32, 0:j0]^T   (MFMA, wave w: rows 16w..)
@@ -1213,26 +1130,6 @@ __global__ void k_fill_int(int *p, int v, int n)
 int g_panel_kernel = 1;
 // trailing-update engine: 0 = FP64 VALU, 1 = MFMA (default)
 int g_update_kernel = -1;
-// Diagonal factors through the persistent server (k_potrf_server): bit 0 for
-// FP64 factorizations, bit 1 for FP32 ones; -1 = not yet read from the
-// environment (DPLASMA_MI_POTRF_SERVER, default 0: off).  The server waits
-// on kernels of another stream; HIP maps streams onto a few hardware queues,
-// and a kernel queued behind the server in its queue cannot start until the
-// server ends, so with several contexts (or other streams) the server and
-// its feeder can wait on each other until the bounded wait gives up.
-int g_potrf_server = -1;
-// bound on one wait of the server protocol, ticks of the 100 MHz counter (10 s)
-long long g_sv_wait_ticks = 1000000000LL;
-
-template <class T> bool use_server()
-{
-    if (g_potrf_server < 0) {
-        const char *e = getenv("DPLASMA_MI_POTRF_SERVER");
-        g_potrf_server = e ? atoi(e) & 3 : 0;
-    }
-    return (g_potrf_server >> (sizeof(T) == 8 ? 0 : 1)) & 1;
-}
-
 // FP32 trailing update of mi_spotrf_tiles and the FP32 step launches: 0 =
 // FP32 MFMA (default), 1 = BF16 MFMA, two-term split (the mixed solve passes
 // its choice as an argument instead)
@@ -1247,8 +1144,6 @@ template <> struct KN<double> {
     static constexpr const char *trsm = "dpotrf_trsm_panel";
     static constexpr const char *update = "dpotrf_update";
     static constexpr const char *update_la = "dpotrf_update_la";
-    static constexpr const char *potrf_server = "dpotrf_potrf_server";
-    static constexpr const char *potrf_wait = "dpotrf_potrf_wait";
 };
 template <> struct KN<float> {
     static constexpr const char *potrf = "spotrf_potrf_diag";
@@ -1257,8 +1152,6 @@ template <> struct KN<float> {
     static constexpr const char *update_la = "spotrf_update_la";
     static constexpr const char *update_bf16 = "spotrf_update_bf16x3";
     static constexpr const char *update_bf16_la = "spotrf_update_bf16x3_la";
-    static constexpr const char *potrf_server = "spotrf_potrf_server";
-    static constexpr const char *potrf_wait = "spotrf_potrf_wait";
 };
 
 template <class T> bool valu_panel() { return std::is_same<T, double>::value && g_panel_kernel == 0; }
@@ -1410,9 +1303,6 @@ int prepare_potrf(int NB)
     if (lds > 65536) {
         MI_CHECK(hipFuncSetAttribute(!valu_panel<T>() ? (const void *)k_potrf_tile<T> : (const void *)k_potrf_diag,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-        if (!valu_panel<T>())
-            MI_CHECK(hipFuncSetAttribute((const void *)k_potrf_server<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         lds));
     }
     return 0;
 }
@@ -1453,57 +1343,15 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws,
     c->ev_next = 0;
     size_t ring_next = 0;
     auto ev = [&]() -> hipEvent_t { return ring ? c->ev_ring[ring_next++ % c->ev_ring.size()] : next_event(c); };
-    // the diagonal factors through the server (not for the allocation-free
-    // entry, whose flags would need uncached memory of their own)
-    const bool server = !ring && !valu_panel<T>() && use_server<T>() && MT > 1;
-    int *flags = nullptr;
-    if (server) {
-        if ((rc = ws_uncached(c, c->sv_flags, 4 * sizeof(int)))) return rc;
-        flags = (int *)c->sv_flags.p;
-    }
-    int RC = 0;
-    const int lds = potrf_lds_bytes<T>(NB, &RC);
-    // POTRF(k): a launch, or a post to the server and a wait for it
     auto diag = [&](int k) -> int {
-        T *w = ws + (k & 1) * wsz;
-        if (!server) return launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k, w);
-        k_sv_post<<<1, 64, 0, c->s_panel>>>(flags, k);
-        MI_LAUNCH(c, c->s_panel, KN<T>::potrf_wait, 0.0, 0.0,
-                  k_sv_wait<<<1, 64, 0, c->s_panel>>>(flags, k, g_sv_wait_ticks));
-        return 0;
+        return launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k, ws + (k & 1) * wsz);
     };
 
     MI_CHECK(hipEventRecord(c->t0, c->s_main));
     k_fill_int<<<1, 64, 0, c->s_main>>>(c->d_info, 0, 1);
-    if (server) k_sv_reset<<<1, 64, 0, c->s_main>>>(flags);
     hipEvent_t start = ev();
     MI_CHECK(hipEventRecord(start, c->s_main));
     MI_CHECK(hipStreamWaitEvent(c->s_panel, start, 0));
-    hipEvent_t aux_done = nullptr;
-    // an error return after the server started stops it (abort word, then a
-    // wait for its stream), so that no server outlives its call
-    struct ServerGuard {
-        Ctx *c;
-        int *flags;
-        ~ServerGuard()
-        {
-            if (!flags) return;
-            const int one = 1;
-            (void)hipMemcpy(flags + 2, &one, sizeof(int), hipMemcpyHostToDevice);
-            (void)hipStreamSynchronize(c->s_aux);
-        }
-    } guard{c, nullptr};
-    if (server) {
-        MI_CHECK(hipStreamWaitEvent(c->s_aux, start, 0));
-        double fl = 0.0;
-        for (int k = 0; k < MT; k++) fl += std::pow((double)tile_rows(N, NB, MT, k), 3.0) / 3.0;
-        MI_LAUNCH(c, c->s_aux, KN<T>::potrf_server, fl, 0.0,
-                  k_potrf_server<T><<<1, 512, lds, c->s_aux>>>(dT, N, NB, MT, RC, 0, flags, ws, (int)wsz,
-                                                               c->d_info, g_sv_wait_ticks));
-        guard.flags = flags;
-        aux_done = ev();
-        MI_CHECK(hipEventRecord(aux_done, c->s_aux));
-    }
 
     // panel 0
     if ((rc = diag(0))) return rc;
@@ -1533,18 +1381,10 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws,
         pk = pk_next;
     }
     MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
-    if (aux_done) MI_CHECK(hipStreamWaitEvent(c->s_main, aux_done, 0));
     MI_CHECK(hipEventRecord(c->t1, c->s_main));
-    int h_info = 0, h_flags[3] = {0, 0, 0};
+    int h_info = 0;
     MI_CHECK(hipMemcpyAsync(&h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
-    if (server) MI_CHECK(hipMemcpyAsync(h_flags, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c->s_main));
     if ((rc = mi_synchronize(static_cast<mi_ctx_t *>(c)))) return rc;
-    guard.flags = nullptr;                        // the server has finished
-    if (server && (h_flags[2] || h_flags[1] != MT - 1)) {
-        set_error("tile Cholesky: the diagonal-factor server stopped at step %d of %d (wait limit)", h_flags[1],
-                  MT - 1);
-        return -5;
-    }
     float ms = 0.f;
     MI_CHECK(hipEventElapsedTime(&ms, c->t0, c->t1));
     c->last_ms = ms;
@@ -1596,7 +1436,6 @@ int step_update(mi_ctx_t *c, int N, int NB, T *dT, int k, int n_begin, int n_end
 }  // namespace
 
 namespace mi {
-void set_server_wait_ticks(long long ticks) { g_sv_wait_ticks = ticks; }
 
 // Single tile tasks on the packed lower image, enqueued on s (the bodies of
 // the four task classes of src/zpotrf_L.jdf:91-633, one tile each).
@@ -1675,13 +1514,6 @@ int mi_set_spotrf_update(int which)
     return 0;
 }
 
-int mi_set_potrf_server(int mask)
-{
-    if (mask < 0 || mask > 3) { set_error("server mask must be 0..3 (bit 0 FP64, bit 1 FP32)"); return -2; }
-    g_potrf_server = mask;
-    return 0;
-}
-
 int mi_set_update_engine(int which)
 {
     if (which != 0 && which != 1) { set_error("update engine must be 0 (VALU) or 1 (MFMA)"); return -2; }
@@ -1747,9 +1579,9 @@ int mi_dpotrf_tiles_noalloc(mi_ctx_t *c, int N, int NB, double *dT, int *info)
                   hipGetErrorString(e));
         return -1;
     }
-    // per-step POTRF launches and the context's fixed event ring (the
-    // diagonal-factor server and the event pool would allocate); the
-    // profiler creates events on demand, so it is off for this call
+    // the context's fixed event ring instead of its event pool, which grows
+    // on demand; the profiler creates events on demand, so it is off for
+    // this call
     const bool prof = c->prof;
     c->prof = false;
     const int rc = potrf_tiles<double>(c, N, NB, dT, info, false, ptr, true);

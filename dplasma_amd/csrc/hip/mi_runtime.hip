@@ -25,8 +25,7 @@ void *ws_get(Ctx *c, int id, size_t bytes)
     if (b.p && b.bytes >= bytes) return b.p;
     if (b.p) {
         // the old buffer may still be read by work queued on this context
-        if (hipStreamSynchronize(c->s_panel) != hipSuccess || hipStreamSynchronize(c->s_main) != hipSuccess ||
-            hipStreamSynchronize(c->s_aux) != hipSuccess) {
+        if (hipStreamSynchronize(c->s_panel) != hipSuccess || hipStreamSynchronize(c->s_main) != hipSuccess) {
             set_error("workspace %d: stream synchronization failed", id);
             return nullptr;
         }
@@ -169,7 +168,6 @@ int mi_ctx_create(int device, mi_ctx_t **out)
     // queue arbitration, not on stream priorities
     MI_CHECK(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
     MI_CHECK(hipStreamCreateWithFlags(&c->s_panel, hipStreamNonBlocking));
-    MI_CHECK(hipStreamCreateWithFlags(&c->s_aux, hipStreamNonBlocking));
     MI_CHECK(hipMalloc(&c->d_info, 64 * sizeof(int)));
     c->scratch_bytes = 1 << 20;
     MI_CHECK(hipMalloc(&c->d_scratch, c->scratch_bytes));
@@ -191,7 +189,6 @@ void mi_ctx_destroy(mi_ctx_t *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->s_main);
     (void)hipStreamSynchronize(c->s_panel);
-    (void)hipStreamSynchronize(c->s_aux);
     mi::prof_collect(c);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto e : c->prof_free) (void)hipEventDestroy(e);
@@ -206,18 +203,15 @@ void mi_ctx_destroy(mi_ctx_t *c)
         std::lock_guard<std::mutex> g(mi::g_retired_mu);
         if (c->wv_ticket.p) mi::g_retired.push_back(c->wv_ticket.p);
         if (c->wv_stage.p) mi::g_retired.push_back(c->wv_stage.p);
-        if (c->sv_flags.p) mi::g_retired.push_back(c->sv_flags.p);
     }
     (void)hipStreamDestroy(c->s_main);
     (void)hipStreamDestroy(c->s_panel);
-    (void)hipStreamDestroy(c->s_aux);
     delete c;
 }
 
 int mi_synchronize(mi_ctx_t *c)
 {
     MI_CHECK(hipStreamSynchronize(c->s_panel));
-    MI_CHECK(hipStreamSynchronize(c->s_aux));
     MI_CHECK(hipStreamSynchronize(c->s_main));
     c->ev_next = 0;
     if (c->prof) mi::prof_collect(c);

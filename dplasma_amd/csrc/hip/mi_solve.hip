@@ -1191,7 +1191,6 @@ extern "C" int mi_set_solve_wait_limit(double seconds)
 {
     if (!(seconds >= 0.0) || seconds > 3600.0) { set_error("wait limit must be in [0, 3600] s"); return -2; }
     g_wave_wait_ticks = (long long)(seconds * 1e8);
-    set_server_wait_ticks(g_wave_wait_ticks);
     return 0;
 }
 

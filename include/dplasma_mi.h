@@ -136,9 +136,8 @@ int mi_dpotrs_tiles(mi_ctx_t *ctx, int N, int NB, const double *dT, int NRHS, do
  * through uncached device memory) whenever NB % 32 == 0; 0 = one
  * diagonal-solve and one update launch per tile.  Process-wide. */
 int mi_set_solve_engine(int which);
-/* Bound on one hand-off wait between workgroups -- the wavefront solve's
- * and the diagonal-factor server's (mi_set_potrf_server) -- in seconds
- * (default 10; 0 gives up at once).  A wait that exceeds it ends the
+/* Bound on one hand-off wait between workgroups of the wavefront solve, in
+ * seconds (default 10; 0 gives up at once).  A wait that exceeds it ends the
  * launches early: the blocking call then returns -5 and its outputs are not
  * valid.  Process-wide. */
 int mi_set_solve_wait_limit(double seconds);
@@ -182,17 +181,6 @@ int mi_dpotrf_step_potrf(mi_ctx_t *ctx, int N, int NB, double *dT, int k, int *i
  * MFMA 16x16x4 (default; two resident waves per SIMD keep it at full rate).
  * Process-wide. */
 int mi_set_update_engine(int which);
-/* Diagonal factors of mi_dpotrf_tiles / mi_spotrf_tiles through one
- * persistent workgroup that holds a CU for the whole factorization and runs
- * POTRF(k) as the panel stream posts it (bit 0: FP64, bit 1: FP32), instead
- * of one launch per step, which has to wait for a CU to empty beside the
- * trailing update.  Its waits are bounded like the solve's (a factorization
- * whose server stops returns -5).  Off by default: HIP maps streams onto a
- * few hardware queues, so with more than one context (or other busy
- * streams) a kernel the server waits for can be queued behind it, and the
- * factorization then ends with -5 after the wait limit.  Process-wide;
- * default 0, or the environment variable DPLASMA_MI_POTRF_SERVER. */
-int mi_set_potrf_server(int mask);
 /* Engine of POTRF(k) and the panel TRSM: 0 = VALU kernels, 1 = MFMA panel
  * updates with register/LDS diagonal blocks (default).  Process-wide. */
 int mi_set_panel_engine(int which);
