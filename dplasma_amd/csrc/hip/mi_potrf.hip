@@ -673,24 +673,32 @@ __device__ inline void decode_block(int L, int nsub, int n_begin, int m_off, int
     *bn = s - (*bm) * nsub;
 }
 
-// The FP64 MFMA pipe of a gfx950 SIMD needs two resident waves to stay
-// busy (one wave alone issues at half rate), so the kernel is held to 256
-// registers per lane (128 accumulators) and two workgroups share each CU.
-template <class T, int BMX>
-__global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int N, int NB, int MT, int k,
-                                                       int n_begin, int m_off, int nsub)
+// The FP64 MFMA pipe of a gfx950 SIMD issues at full rate only with two
+// waves ready on it (one wave alone issues at half rate).  The workgroup's
+// waves form a WR x WC grid over its BMX x BMX block, each wave a
+// (BMX/WR) x (BMX/WC) sub-block of 16x16 MFMA tiles.  Two workgroups share
+// each CU: WR x WC = 2 x 2 holds 256 registers per lane (two waves per
+// SIMD); 4 x 2 holds 128 (four waves per SIMD), so that a workgroup waiting
+// at a barrier or on memory still leaves two waves per SIMD issuing.
+template <class T, int BMX, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC, 2) void k_update_mfma(T *__restrict__ Tt, int N, int NB, int MT, int k,
+                                                                int n_begin, int m_off, int nsub)
 {
     // rows padded by 128 bytes: the four lane groups of a fragment read (k
     // rows 4s + l/16) then fall on alternating halves of the LDS banks
     constexpr int LP = BMX + 128 / (int)sizeof(T);
+    constexpr int NT = 64 * WR * WC;                  // threads
     __shared__ T As[2][BK][LP];
     __shared__ T Bs[2][BK][LP];
     typedef typename MF<T>::v4 V4;
     typedef typename MF<T>::v2 V2;
-    constexpr int TI = BMX / 32;                      // 16x16 MFMA tiles per wave and dimension
-    constexpr int HB = BMX / 2;                       // rows (columns) per wave
+    constexpr int HR = BMX / WR, HC = BMX / WC;       // rows / columns per wave
+    constexpr int TR = HR / 16, TC = HC / 16;         // 16x16 MFMA tiles per wave
+    constexpr int PR = BMX / 2;                       // element pairs per k row of a panel
+    constexpr int SP = BK * PR / NT;                  // pairs staged per thread and operand
+    static_assert(TR % 2 == 0 && SP * NT == BK * PR, "wave grid does not fit the block");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WC, wn = wave % WC;
 
     // XCD-aware remap: consecutive workgroup ids land on different XCDs
     // (round-robin), so give each XCD a contiguous range of blocks; the
@@ -713,29 +721,29 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
     const T *Bp = Tt + tile_index(MT, n, k) * (int64_t)NB * NB;
     T *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
 
-    V4 acc[TI][TI];
+    V4 acc[TR][TC];
 #pragma unroll
-    for (int i = 0; i < TI; i++)
+    for (int i = 0; i < TR; i++)
 #pragma unroll
-        for (int j = 0; j < TI; j++) acc[i][j] = V4{T(0), T(0), T(0), T(0)};
+        for (int j = 0; j < TC; j++) acc[i][j] = V4{T(0), T(0), T(0), T(0)};
 
-    // global -> register staging in element pairs: pair e = tid + 256*i,
-    // k row kk = e / HB, rows 2*(e % HB) and 2*(e % HB) + 1
-    V2 ra[TI], rb[TI];
+    // global -> register staging in element pairs: pair e = tid + NT*i,
+    // k row kk = e / PR, rows 2*(e % PR) and 2*(e % PR) + 1
+    V2 ra[SP], rb[SP];
     auto gload = [&](int k0) {
         if (full) {
 #pragma unroll
-            for (int i = 0; i < TI; i++) {
-                const int e = tid + 256 * i;
-                const int kk = e / HB, r = (e % HB) * 2;
+            for (int i = 0; i < SP; i++) {
+                const int e = tid + NT * i;
+                const int kk = e / PR, r = (e % PR) * 2;
                 ra[i] = *(const V2 *)(Ap + (int64_t)(k0 + kk) * NB + r_base + r);
                 rb[i] = *(const V2 *)(Bp + (int64_t)(k0 + kk) * NB + c_base + r);
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < TI; i++) {
-                const int e = tid + 256 * i;
-                const int kk = e / HB, r = (e % HB) * 2;
+            for (int i = 0; i < SP; i++) {
+                const int e = tid + NT * i;
+                const int kk = e / PR, r = (e % PR) * 2;
                 const int kg = k0 + kk;
                 T a0 = T(0), a1 = T(0), b0 = T(0), b1 = T(0);
                 if (kg < NB) {
@@ -753,9 +761,9 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < TI; i++) {
-            const int e = tid + 256 * i;
-            const int kk = e / HB, r = (e % HB) * 2;
+        for (int i = 0; i < SP; i++) {
+            const int e = tid + NT * i;
+            const int kk = e / PR, r = (e % PR) * 2;
             *(V2 *)&As[buf][kk][r] = V2{-ra[i].x, -ra[i].y};     // -A: the MFMA accumulates C - A B^T
             *(V2 *)&Bs[buf][kk][r] = rb[i];
         }
@@ -770,13 +778,13 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
     const bool pre = full && !((m == n) && (bm == bn));
     if (pre) {
 #pragma unroll
-        for (int p = 0; p < TI / 2; p++)
+        for (int p = 0; p < TR / 2; p++)
 #pragma unroll
-            for (int j = 0; j < TI; j++) {
-                const int row = r_base + wm * HB + 32 * p + 2 * fr0;
+            for (int j = 0; j < TC; j++) {
+                const int row = r_base + wm * HR + 32 * p + 2 * fr0;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const V2 c = *(const V2 *)&Cp[(int64_t)(c_base + wn * HB + j * 16 + MF<T>::row(q, fk0)) * NB + row];
+                    const V2 c = *(const V2 *)&Cp[(int64_t)(c_base + wn * HC + j * 16 + MF<T>::row(q, fk0)) * NB + row];
                     acc[2 * p][j][q] = c.x;
                     acc[2 * p + 1][j][q] = c.y;
                 }
@@ -792,19 +800,19 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
         if (st + 1 < nstage) gload((st + 1) * BK);
 #pragma unroll
         for (int kq = 0; kq < BK / 4; kq++) {
-            T a[TI], b[TI];
+            T a[TR], b[TC];
 #pragma unroll
-            for (int p = 0; p < TI / 2; p++) {
-                const V2 v = *(const V2 *)&As[buf][kq * 4 + fk][wm * HB + 32 * p + 2 * fr];
+            for (int p = 0; p < TR / 2; p++) {
+                const V2 v = *(const V2 *)&As[buf][kq * 4 + fk][wm * HR + 32 * p + 2 * fr];
                 a[2 * p] = v.x;
                 a[2 * p + 1] = v.y;
             }
 #pragma unroll
-            for (int j = 0; j < TI; j++) b[j] = Bs[buf][kq * 4 + fk][wn * HB + j * 16 + fr];
+            for (int j = 0; j < TC; j++) b[j] = Bs[buf][kq * 4 + fk][wn * HC + j * 16 + fr];
 #pragma unroll
-            for (int i = 0; i < TI; i++)
+            for (int i = 0; i < TR; i++)
 #pragma unroll
-                for (int j = 0; j < TI; j++)
+                for (int j = 0; j < TC; j++)
                     acc[i][j] = MF<T>::mfma(b[j], a[i], acc[i][j]);   // C^T: lanes walk rows
         }
         if (st + 1 < nstage) lstore(buf ^ 1);
@@ -816,13 +824,13 @@ __global__ __launch_bounds__(256, 2) void k_update_mfma(T *__restrict__ Tt, int 
     // tile: a store walks row pairs
     const bool diag = (m == n) && (bm == bn);
 #pragma unroll
-    for (int p = 0; p < TI / 2; p++) {
-        const int rr = r_base + wm * HB + 32 * p + 2 * fr;
+    for (int p = 0; p < TR / 2; p++) {
+        const int rr = r_base + wm * HR + 32 * p + 2 * fr;
 #pragma unroll
-        for (int j = 0; j < TI; j++)
+        for (int j = 0; j < TC; j++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int col = c_base + wn * HB + j * 16 + MF<T>::row(q, fk);
+                const int col = c_base + wn * HC + j * 16 + MF<T>::row(q, fk);
                 if (pre) {
                     *(V2 *)&Cp[(int64_t)col * NB + rr] = V2{acc[2 * p][j][q], acc[2 * p + 1][j][q]};
                 } else {
@@ -1130,6 +1138,9 @@ __global__ void k_fill_int(int *p, int v, int n)
 int g_panel_kernel = 1;
 // trailing-update engine: 0 = FP64 VALU, 1 = MFMA (default)
 int g_update_kernel = -1;
+// waves per workgroup of the 128x128 MFMA update: 4 (2 x 2) or 8 (4 x 2);
+// 0 = not yet read from the environment (DPLASMA_MI_UPDATE_WAVES)
+int g_update_waves = 0;
 // FP32 trailing update of mi_spotrf_tiles and the FP32 step launches: 0 =
 // FP32 MFMA (default), 1 = BF16 MFMA, two-term split (the mixed solve passes
 // its choice as an argument instead)
@@ -1241,6 +1252,10 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
         const char *e = getenv("DPLASMA_MI_UPDATE");
         g_update_kernel = (e && !strcmp(e, "valu")) ? 0 : 1;
     }
+    if (!g_update_waves) {
+        const char *e = getenv("DPLASMA_MI_UPDATE_WAVES");
+        g_update_waves = (e && atoi(e) == 4) ? 4 : 8;
+    }
     const bool valu = std::is_same<T, double>::value && g_update_kernel == 0;
     const bool bf16 = std::is_same<T, float>::value && bf16_split;
     const bool small = grid < 512 && !valu && !bf16;
@@ -1268,10 +1283,16 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
                                                                  nsub));
     else if (small)
         MI_LAUNCH(c, s, la ? KN<T>::update_la : KN<T>::update, fl, by,
-                  k_update_mfma<T, 64><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
+                  k_update_mfma<T, 64, 2, 2><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off,
+                                                                            nsub));
+    else if (g_update_waves == 8)
+        MI_LAUNCH(c, s, la ? KN<T>::update_la : KN<T>::update, fl, by,
+                  k_update_mfma<T, BM, 4, 2><<<(unsigned)grid, 512, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off,
+                                                                            nsub));
     else
         MI_LAUNCH(c, s, la ? KN<T>::update_la : KN<T>::update, fl, by,
-                  k_update_mfma<T, BM><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off, nsub));
+                  k_update_mfma<T, BM, 2, 2><<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, n_begin, m_off,
+                                                                            nsub));
     return 0;
 }
 
