@@ -609,14 +609,19 @@ __global__ __launch_bounds__(256) void k_wave_link(const T *__restrict__ Tt, int
     for (int e = 0; e < 64; e++) Mg[TRANS ? (i0 + 2 * e) * 128 + k : k * 128 + i0 + 2 * e] = acc[e];
 }
 
-// Index mappings of the wavefront solve.  A wave owns 32 rows of its block;
-// MFMA row i (0..15) of product t (0, 1) is row wv_row(t, i) of them, and
-// element kq (0..7) of lane group f (0..3) of a 32-column slice of the
-// contraction is column wv_k(kq, f) of it.  Forward, two consecutive rows
-// belong to one lane (2 i + t), so that the column-major L operands load as
-// pairs; backward, four consecutive columns belong to one lane, so that the
-// rows of L (the columns of L^T) load as runs of four.
-template <bool TRANS> __device__ inline int wv_row(int t, int i) { return TRANS ? 16 * t + i : 2 * i + t; }
+// Index mappings of the wavefront solve.  The NW waves of a workgroup own
+// 128 / NW rows each, as TPW = 8 / NW MFMA row tiles of 16 rows.  With two
+// tiles (NW = 4), MFMA row i (0..15) of tile t (0, 1) is row wv_row(t, i) of
+// the wave's rows: forward, two consecutive rows belong to one lane (2 i + t),
+// so that the column-major L operands load as pairs.  With one tile (NW = 8)
+// row i is row i.  Element kq (0..7) of lane group f (0..3) of a 32-column
+// slice of the contraction is column wv_k(kq, f) of it: backward, four
+// consecutive columns belong to one lane, so that the rows of L (the
+// columns of L^T) load as runs of four.
+template <bool TRANS, int TPW> __device__ inline int wv_row(int t, int i)
+{
+    return TPW == 1 ? i : (TRANS ? 16 * t + i : 2 * i + t);
+}
 template <bool TRANS> __device__ inline int wv_k(int kq, int f) { return TRANS ? 16 * (kq >> 2) + 4 * f + (kq & 3) : 4 * kq + f; }
 
 // Staging position of X(row R of the block, right-hand side j).  FP64: [row]
@@ -637,15 +642,19 @@ template <class T, bool TRANS> __device__ inline int wv_xs(int R, int j)
 // into the abort word flags[1] and goes on with whatever it read; every other
 // wait ends as soon as it sees that word set, so the launch always drains and
 // the host reports the failure.
-template <class T, bool TRANS>
-__global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, int N, int NB, int MT,
-                                                     const T *__restrict__ W128, const T *__restrict__ Mlink,
-                                                     T *__restrict__ B, int ldb, int nr, T *__restrict__ xs,
-                                                     int *__restrict__ ticket, long long wait_ticks)
+template <class T, bool TRANS, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ Tt, int N, int NB, int MT,
+                                                         const T *__restrict__ W128, const T *__restrict__ Mlink,
+                                                         T *__restrict__ B, int ldb, int nr, T *__restrict__ xs,
+                                                         int *__restrict__ ticket, long long wait_ticks)
 {
     typedef typename MF<T>::v4 V4;
     typedef typename MF<T>::v2 V2;
     typedef typename WvSent<T>::U U;
+    constexpr int NT = 64 * NW;                   // threads
+    constexpr int RW = WV_RB / NW;                // rows per wave
+    constexpr int TPW = RW / 16;                  // MFMA row tiles per wave
+    static_assert(TPW == 1 || TPW == 2, "16 or 32 rows per wave");
     __shared__ T Ws[128 * WV_WS];    // a-operands of the diagonal product: Ws[k * WV_WS + i] = op(W)(i, k)
     __shared__ T Ys[128][17];
     __shared__ int s_g;
@@ -660,60 +669,70 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
     __syncthreads();
     const int g = TRANS ? nwg - 1 - s_g : s_g;
     const int r0 = g * WV_RB;
-    const int rw = r0 + 32 * wave;               // this wave's 32 rows
+    const int rw = r0 + RW * wave;               // this wave's rows
     // Loads below always use in-bounds addresses (indices clamped into the
     // tile or matrix) and mask the value afterwards, so they issue together
     // instead of one branch and one wait each.
     {
         const T *Wg = W128 + (int64_t)g * 16384;
-        for (int e0 = 0; e0 < 16384; e0 += 256 * 16) {
+        for (int e0 = 0; e0 < 16384; e0 += NT * 16) {
             T v[16];
 #pragma unroll
             for (int u = 0; u < 16; u++) {
-                const int e = e0 + 256 * u + tid, i = e & 127, k = e >> 7;
+                const int e = e0 + NT * u + tid, i = e & 127, k = e >> 7;
                 v[u] = TRANS ? Wg[i * 128 + k] : Wg[k * 128 + i];
             }
 #pragma unroll
             for (int u = 0; u < 16; u++) {
-                const int e = e0 + 256 * u + tid, i = e & 127, k = e >> 7;
+                const int e = e0 + NT * u + tid, i = e & 127, k = e >> 7;
                 Ws[k * WV_WS + i] = v[u];
             }
         }
     }
     const int frc = min(fr, max(nr, 1) - 1);
-    V4 acc[2];
+    V4 acc[TPW];
 #pragma unroll
-    for (int t = 0; t < 2; t++)
+    for (int t = 0; t < TPW; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int row = rw + wv_row<TRANS>(t, MF<T>::row(q, fk));
+            const int row = rw + wv_row<TRANS, TPW>(t, MF<T>::row(q, fk));
             const T v = B[(int64_t)frc * ldb + min(row, N - 1)];
             acc[t][q] = (row < N && fr < nr) ? v : T(0);
         }
 
     // a-operands of one other block h: op(L)(my row wv_row(t, fr), column
     // 32 hb + wv_k(kq, fk) of the block)
-    T la[4][8][2];
+    T la[4][8][TPW];
     auto load_a = [&](int h) {
 #pragma unroll
         for (int hb = 0; hb < 4; hb++) {
             const int c0 = h * WV_RB + 32 * hb;
             const int cc = min(c0, N - 1) & ~31;     // a valid 32-aligned block when c0 >= N
-            const int rr = min(rw, N - 1) & ~31;
+            const int rr = min(rw, N - 1) & ~(RW - 1);
             if (!TRANS) {
-                const T *pA = sym_elem(Tt, MT, NB, rr, cc) + 2 * fr;
+                if constexpr (TPW == 2) {
+                    const T *pA = sym_elem(Tt, MT, NB, rr, cc) + 2 * fr;
 #pragma unroll
-                for (int kq = 0; kq < 8; kq++) {
-                    const int kk = 4 * kq + fk;
-                    const V2 v = *(const V2 *)(pA + (int64_t)kk * NB);
-                    const bool ck = c0 < N && c0 + kk < N;
-                    la[hb][kq][0] = (ck && rw + 2 * fr < N) ? v.x : T(0);
-                    la[hb][kq][1] = (ck && rw + 2 * fr + 1 < N) ? v.y : T(0);
+                    for (int kq = 0; kq < 8; kq++) {
+                        const int kk = 4 * kq + fk;
+                        const V2 v = *(const V2 *)(pA + (int64_t)kk * NB);
+                        const bool ck = c0 < N && c0 + kk < N;
+                        la[hb][kq][0] = (ck && rw + 2 * fr < N) ? v.x : T(0);
+                        la[hb][kq][1] = (ck && rw + 2 * fr + 1 < N) ? v.y : T(0);
+                    }
+                } else {
+                    const T *pA = sym_elem(Tt, MT, NB, rr, cc) + fr;
+#pragma unroll
+                    for (int kq = 0; kq < 8; kq++) {
+                        const int kk = 4 * kq + fk;
+                        const T v = pA[(int64_t)kk * NB];
+                        la[hb][kq][0] = (c0 < N && c0 + kk < N && rw + fr < N) ? v : T(0);
+                    }
                 }
             } else {
                 const T *pA = sym_elem(Tt, MT, NB, cc, rr) + 4 * fk;
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
+                for (int t = 0; t < TPW; t++) {
                     const int i = 16 * t + fr;
 #pragma unroll
                     for (int kg = 0; kg < 2; kg++) {
@@ -730,7 +749,7 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
             }
         }
     };
-    // a-operands of the link matrix: M_g(32 w + wv_row(t, fr), 32 hb + wv_k(kq, fk)),
+    // a-operands of the link matrix: M_g(RW w + wv_row(t, fr), 32 hb + wv_k(kq, fk)),
     // column-major forward, row-major backward
     auto load_m = [&]() {
         const T *Mg = Mlink + (int64_t)g * 16384;
@@ -739,16 +758,20 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
             if (!TRANS) {
 #pragma unroll
                 for (int kq = 0; kq < 8; kq++) {
-                    const V2 v = *(const V2 *)(Mg + (32 * hb + 4 * kq + fk) * 128 + 32 * wave + 2 * fr);
-                    la[hb][kq][0] = v.x;
-                    la[hb][kq][1] = v.y;
+                    if constexpr (TPW == 2) {
+                        const V2 v = *(const V2 *)(Mg + (32 * hb + 4 * kq + fk) * 128 + RW * wave + 2 * fr);
+                        la[hb][kq][0] = v.x;
+                        la[hb][kq][1] = v.y;
+                    } else {
+                        la[hb][kq][0] = Mg[(32 * hb + 4 * kq + fk) * 128 + RW * wave + fr];
+                    }
                 }
             } else {
 #pragma unroll
-                for (int t = 0; t < 2; t++)
+                for (int t = 0; t < TPW; t++)
 #pragma unroll
                     for (int kg = 0; kg < 2; kg++) {
-                        const T *p = Mg + (32 * wave + 16 * t + fr) * 128 + 32 * hb + 16 * kg + 4 * fk;
+                        const T *p = Mg + (RW * wave + 16 * t + fr) * 128 + 32 * hb + 16 * kg + 4 * fk;
                         const V2 v0 = *(const V2 *)p, v1 = *(const V2 *)(p + 2);
                         la[hb][4 * kg + 0][t] = v0.x;
                         la[hb][4 * kg + 1][t] = v0.y;
@@ -759,7 +782,7 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
         }
     };
     // -X(block h) as b-operands, once every value has been stored: for the
-    // last block (fin) lanes 0..3 first poll one value of each producer
+    // last block (fin) lanes 0..NW-1 first poll one value of each producer
     // wave; then the wave loads its values, and loads again while any of
     // them is still the sentinel
     // true once this wait has to end without its values: the abort word is
@@ -779,8 +802,8 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
         const T *xh = xs + (int64_t)h * (WV_RB * RHSB);
         const long long t_end = (long long)__builtin_amdgcn_s_memrealtime() + wait_ticks;
         if (wait_ticks == 0) (void)give_up(t_end);   // a zero budget ends every wait at once
-        if (fin && lane < 4)
-            while (wv_load(xh + wv_xs<T, TRANS>(32 * lane + 31, RHSB - 1)) == WvSent<T>::v) {
+        if (fin && lane < NW)
+            while (wv_load(xh + wv_xs<T, TRANS>(RW * lane + RW - 1, RHSB - 1)) == WvSent<T>::v) {
                 if (give_up(t_end)) break;
                 __builtin_amdgcn_s_sleep(1);
             }
@@ -834,7 +857,7 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
 #pragma unroll
                 for (int kq = 0; kq < 8; kq++)
 #pragma unroll
-                    for (int t = 0; t < 2; t++) acc[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], acc[t]);
+                    for (int t = 0; t < TPW; t++) acc[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], acc[t]);
             if (s + 2 < nprev) load_a(TRANS ? nwg - 2 - s : s + 1);
         }
     }
@@ -842,20 +865,20 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
 
     // z = op(W_g) y
 #pragma unroll
-    for (int t = 0; t < 2; t++)
+    for (int t = 0; t < TPW; t++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) Ys[32 * wave + wv_row<TRANS>(t, MF<T>::row(q, fk))][fr] = acc[t][q];
+        for (int q = 0; q < 4; q++) Ys[RW * wave + wv_row<TRANS, TPW>(t, MF<T>::row(q, fk))][fr] = acc[t][q];
     __syncthreads();
-    V4 x[2];
+    V4 x[TPW];
 #pragma unroll
-    for (int t = 0; t < 2; t++) x[t] = V4{T(0), T(0), T(0), T(0)};
+    for (int t = 0; t < TPW; t++) x[t] = V4{T(0), T(0), T(0), T(0)};
 #pragma unroll 8
     for (int kq = 0; kq < 32; kq++) {
         const int kk = 4 * kq + fk;
         const T b = Ys[kk][fr];
 #pragma unroll
-        for (int t = 0; t < 2; t++)
-            x[t] = MF<T>::mfma(Ws[kk * WV_WS + 32 * wave + wv_row<TRANS>(t, fr)], b, x[t]);
+        for (int t = 0; t < TPW; t++)
+            x[t] = MF<T>::mfma(Ws[kk * WV_WS + RW * wave + wv_row<TRANS, TPW>(t, fr)], b, x[t]);
     }
     // x_g = z - M_g X(last block)
     if (nprev > 0 && rw < N) {
@@ -866,14 +889,14 @@ __global__ __launch_bounds__(256, 1) void k_trsm_wave(const T *__restrict__ Tt, 
 #pragma unroll
             for (int kq = 0; kq < 8; kq++)
 #pragma unroll
-                for (int t = 0; t < 2; t++) x[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], x[t]);
+                for (int t = 0; t < TPW; t++) x[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], x[t]);
     }
     T *xg = xs + (int64_t)g * (WV_RB * RHSB);
 #pragma unroll
-    for (int t = 0; t < 2; t++)
+    for (int t = 0; t < TPW; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int rb = 32 * wave + wv_row<TRANS>(t, MF<T>::row(q, fk));
+            const int rb = RW * wave + wv_row<TRANS, TPW>(t, MF<T>::row(q, fk));
             wv_store(xg + wv_xs<T, TRANS>(rb, fr), x[t][q]);
             if (r0 + rb < N && fr < nr) B[(int64_t)fr * ldb + r0 + rb] = x[t][q];
         }
@@ -917,6 +940,17 @@ __global__ void k_wave_abort_clear(int *flags)
 // every access reaches memory, so a value stored by one XCD is visible to
 // the others without cache maintenance.
 int g_wave_solve = 1;   // 1: one-launch wavefront solve when NB % 32 == 0; 0: per-tile launches
+// waves per workgroup of the wavefront solve: 4 (32 rows each) or 8 (16
+// rows each); 0 = not yet read from the environment (DPLASMA_MI_SOLVE_WAVES)
+int g_wave_waves = 0;
+int wave_waves()
+{
+    if (!g_wave_waves) {
+        const char *e = getenv("DPLASMA_MI_SOLVE_WAVES");
+        g_wave_waves = (e && atoi(e) == 4) ? 4 : 8;
+    }
+    return g_wave_waves;
+}
 // bound on one hand-off wait of the wavefront solve, in ticks of the 100 MHz
 // real-time counter (default 10 s; 0 = give up at once)
 long long g_wave_wait_ticks = 1000000000LL;
@@ -1058,12 +1092,20 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
             const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
             T *Bj = dB + (int64_t)j0 * ldb;
             k_wave_reset<T><<<(unsigned)std::min<int64_t>((nst + 255) / 256, 1024), 256, 0, s>>>(xs, nst, ticket);
-            MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
-                      (trans == 111
-                       ? k_trsm_wave<T, false><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs, ticket,
-                                                                   g_wave_wait_ticks)
-                       : k_trsm_wave<T, true><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs, ticket,
-                                                                  g_wave_wait_ticks)));
+            if (wave_waves() == 8)
+                MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
+                          (trans == 111
+                           ? k_trsm_wave<T, false, 8><<<nwg, 512, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs,
+                                                                          ticket, g_wave_wait_ticks)
+                           : k_trsm_wave<T, true, 8><<<nwg, 512, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs,
+                                                                         ticket, g_wave_wait_ticks)));
+            else
+                MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
+                          (trans == 111
+                           ? k_trsm_wave<T, false, 4><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs,
+                                                                          ticket, g_wave_wait_ticks)
+                           : k_trsm_wave<T, true, 4><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs,
+                                                                         ticket, g_wave_wait_ticks)));
         }
         return 0;
     }
