@@ -940,17 +940,6 @@ __global__ void k_wave_abort_clear(int *flags)
 // every access reaches memory, so a value stored by one XCD is visible to
 // the others without cache maintenance.
 int g_wave_solve = 1;   // 1: one-launch wavefront solve when NB % 32 == 0; 0: per-tile launches
-// waves per workgroup of the wavefront solve: 4 (32 rows each) or 8 (16
-// rows each); 0 = not yet read from the environment (DPLASMA_MI_SOLVE_WAVES)
-int g_wave_waves = 0;
-int wave_waves()
-{
-    if (!g_wave_waves) {
-        const char *e = getenv("DPLASMA_MI_SOLVE_WAVES");
-        g_wave_waves = (e && atoi(e) == 4) ? 4 : 8;
-    }
-    return g_wave_waves;
-}
 // bound on one hand-off wait of the wavefront solve, in ticks of the 100 MHz
 // real-time counter (default 10 s; 0 = give up at once)
 long long g_wave_wait_ticks = 1000000000LL;
@@ -1092,20 +1081,12 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
             const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
             T *Bj = dB + (int64_t)j0 * ldb;
             k_wave_reset<T><<<(unsigned)std::min<int64_t>((nst + 255) / 256, 1024), 256, 0, s>>>(xs, nst, ticket);
-            if (wave_waves() == 8)
-                MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
-                          (trans == 111
-                           ? k_trsm_wave<T, false, 8><<<nwg, 512, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs,
-                                                                          ticket, g_wave_wait_ticks)
-                           : k_trsm_wave<T, true, 8><<<nwg, 512, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs,
-                                                                         ticket, g_wave_wait_ticks)));
-            else
-                MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
-                          (trans == 111
-                           ? k_trsm_wave<T, false, 4><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs,
-                                                                          ticket, g_wave_wait_ticks)
-                           : k_trsm_wave<T, true, 4><<<nwg, 256, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs,
-                                                                         ticket, g_wave_wait_ticks)));
+            MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
+                      (trans == 111
+                       ? k_trsm_wave<T, false, 8><<<nwg, 512, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs,
+                                                                      ticket, g_wave_wait_ticks)
+                       : k_trsm_wave<T, true, 8><<<nwg, 512, 0, s>>>(dT, N, NB, MT, W128, Mb, Bj, ldb, nr, xs,
+                                                                     ticket, g_wave_wait_ticks)));
         }
         return 0;
     }
