@@ -525,9 +525,12 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="run only the batched uneven Cholesky leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="one untimed factorization, for rocprofv3 runs")
-    ap.add_argument("--profile-kind", default="dpotrf", choices=["dpotrf", "spotrf", "spotrf_bf16x3", "dpotrs"],
+    ap.add_argument("--profile-kind", default="dpotrf",
+                    choices=["dpotrf", "spotrf", "spotrf_bf16x3", "dpotrs", "adjoint", "batched", "batched_fp32"],
                     help="what --profile-only runs: the FP64 factorization, the FP32 factorization (FP32 "
-                         "MFMA or BF16-split updates) of the same matrix rounded, or the FP64 solve (16 RHS)")
+                         "MFMA or BF16-split updates) of the same matrix rounded, the FP64 solve (16 RHS), "
+                         "the adjoint of the FP64 factorization, or one launch of the batched solve of "
+                         "4096 uneven matrices (FP64 or FP32) at the tuned shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the plain copy kernel on 1 GiB (scales HBM counters)")
     args = ap.parse_args()
@@ -557,6 +560,48 @@ def main():
     if args.mixed_only:
         kap = tuple(float(k) for k in args.kappas.split(","))
         print(json.dumps({"mixed_dposv": mixed_leg(dp, N, NB, kappas=kap)}))
+        return
+    if args.profile_only and args.profile_kind.startswith("batched"):
+        import ctypes
+        mi, _ = dp.libs()
+        count, seed = 4096, 11
+        ns = (ctypes.c_int * count)()
+        mi.mi_batch_sizes(seed, count, 32, 1024, ns)
+        fp32 = args.profile_kind == "batched_fp32"
+        with dp.Device(0) as dev:
+            p = ctypes.c_void_p()
+            dev.check(mi.mi_batch_create(dev.h, count, ns, ctypes.byref(p)), "batch_create")
+            ae, be = ctypes.c_int64(), ctypes.c_int64()
+            mi.mi_batch_extent(p, ctypes.byref(ae), ctypes.byref(be))
+            A, B = dev.alloc(ae.value * 8), dev.alloc(be.value * 8)
+            dev.check(mi.mi_dbatch_spd(dev.h, count, ns, seed, A), "batch_spd")
+            dev.check(mi.mi_dplrnt(dev.h, be.value, 1, B, be.value, be.value, 0, 0, seed), "b")
+            dev.check(mi.mi_batch_set_shape(p, 1024, 2), "shape")     # the shape the tuner picks here
+            if fp32:
+                A32, B32 = dev.alloc(ae.value * 4), dev.alloc(be.value * 4)
+                ovf = ctypes.c_int()
+                dev.check(mi.mi_dlag2s_tiles(dev.h, ae.value, A, A32, ctypes.byref(ovf)), "A32")
+                dev.check(mi.mi_dlag2s_tiles(dev.h, be.value, B, B32, ctypes.byref(ovf)), "B32")
+                dev.check(mi.mi_sposv_batched(p, A32, B32, 0), "sposv_batched")
+            else:
+                dev.check(mi.mi_dposv_batched(p, A, B, 0), "dposv_batched")
+            mi.mi_batch_destroy(p)
+        print("profile step done")
+        return
+    if args.profile_only and args.profile_kind == "adjoint":
+        import ctypes
+        mi, _ = dp.libs()
+        with dp.Device(0) as dev:
+            MT = (N + NB - 1) // NB
+            nt = MT * (MT + 1) // 2 * NB * NB
+            dA, dG = dev.alloc(nt * 8), dev.alloc(nt * 8)
+            dev.check(mi.mi_dplgsy_tiles(dev.h, float(N), N, NB, dA, 3872), "dplgsy")
+            info = ctypes.c_int()
+            dev.check(mi.mi_dpotrf_tiles(dev.h, N, NB, dA, ctypes.byref(info)), "dpotrf")
+            assert info.value == 0
+            dev.check(mi.mi_dplgsy_tiles(dev.h, 1.0, N, NB, dG, 11), "Lbar")
+            dev.check(mi.mi_dpotrf_adjoint_tiles(dev.h, N, NB, dA, dG), "adjoint")
+        print("profile step done")
         return
     if args.profile_only and args.profile_kind != "dpotrf":
         import ctypes
@@ -620,15 +665,18 @@ def main():
     assert ctx.dpotrf(dp.LOWER, A) == 0
     prof = ctx.prof()
     ctx.prof_enable(False)
-    # the dominant kernel: the trailing updates on the main stream (one
-    # launch at a time, so the average launch duration has no self-overlap);
-    # the look-ahead column updates on the panel stream are reported beside
+    # the dominant kernel: every trailing-update launch (the main stream's
+    # and the panel stream's look-ahead columns), algorithmic flops over
+    # the union of their launch intervals, so that the overlap of the two
+    # streams counts once; the main-stream launches alone are reported beside
     upd = prof.get(UPDATE_KERNEL, (0, 0.0, 0.0, 0.0))
-    achieved = (upd[2] / upd[0]) / (upd[1] / upd[0] * 1e-3) / 1e12 if upd[0] else 0.0
     upd_la = prof.get(UPDATE_KERNEL + "_la", (0, 0.0, 0.0, 0.0))
     busy_main = ctx.prof_busy_ms(UPDATE_KERNEL)
     busy_all = ctx.prof_busy_ms(UPDATE_KERNEL + "*")
     all_upd_flops = upd[2] + upd_la[2]
+    all_upd_bytes = upd[3] + upd_la[3]
+    n_all = upd[0] + upd_la[0]
+    achieved = all_upd_flops / busy_all / 1e9 if busy_all > 0 else 0.0
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -705,16 +753,18 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TF, 4),
             "traffic": traffic,
-            "launches": upd[0],
-            "flops_per_launch": round(upd[2] / upd[0]) if upd[0] else None,
-            "avg_launch_ms": round(upd[1] / upd[0], 4) if upd[0] else None,
-            "busy_ms": round(busy_main, 3),
-            "all_updates": {
-                "what": "main-stream and look-ahead update launches together: algorithmic flops over "
-                        "the union of their launch intervals (overlaps counted once)",
-                "launches": upd[0] + upd_la[0],
-                "busy_ms": round(busy_all, 3),
-                "achieved": round(all_upd_flops / busy_all / 1e9, 3) if busy_all > 0 else None,
+            "what": "all trailing-update launches (main stream and look-ahead): algorithmic flops over the "
+                    "union of their launch intervals (overlaps counted once); traffic = measured HBM bytes "
+                    "per launch averaged over the same launches",
+            "launches": n_all,
+            "flops_per_launch": round(all_upd_flops / n_all) if n_all else None,
+            "algorithmic_bytes_per_launch": round(all_upd_bytes / n_all) if n_all else None,
+            "busy_ms": round(busy_all, 3),
+            "main_stream": {
+                "what": "the bulk updates on the main stream alone (one launch at a time)",
+                "launches": upd[0],
+                "busy_ms": round(busy_main, 3),
+                "achieved": round(upd[2] / busy_main / 1e9, 3) if busy_main > 0 else None,
             },
             "isolated": iso,
         },

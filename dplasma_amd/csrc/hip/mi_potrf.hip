@@ -23,6 +23,7 @@
 // The host loop runs the panel (POTRF+TRSM of step k+1) on a second stream
 // as soon as column k+1 has received its step-k update, so it overlaps the
 // rest of the step-k trailing update (one-panel look-ahead).
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1264,15 +1265,19 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
         grid = ntiles * nsub * nsub;
     }
     if (grid > 0x7fffffff) { set_error("update grid too large"); return -3; }
-    // algorithmic flops / bytes of the covered tiles
+    // algorithmic flops / bytes of the covered tiles: each C tile read and
+    // written once, each panel tile (m, k) the launch needs read once
     double fl = 0.0, by = 0.0;
     int64_t left = ntiles;
+    int m_lo = MT;
     for (int n = n_begin; n < MT && left > 0; n++)
         for (int m = (n == n_begin ? n + m_off : n); m < MT && left > 0; m++, left--) {
             const double tm = tile_rows(N, NB, MT, m);
             fl += (m == n) ? (double)NB * tm * (tm + 1.0) : 2.0 * tm * NB * NB;
-            by += (2.0 * tm * NB + NB * NB) * sizeof(T);
+            by += 2.0 * tm * NB * sizeof(T);
+            m_lo = std::min(m_lo, n);
         }
+    for (int m = m_lo; m < MT; m++) by += (double)tile_rows(N, NB, MT, m) * NB * sizeof(T);
     if (valu)
         MI_LAUNCH(c, s, "dpotrf_update_valu", fl, by,
                   k_update_valu<<<(unsigned)grid, 256, 0, s>>>((double *)dT, N, NB, MT, k, n_begin, m_off,
