@@ -117,6 +117,7 @@ def libs():
     _sig(M, "mi_dadd_diag", i, [vp, i, vp, i, d])
     _sig(M, "mi_dpotrf_tiles", i, [vp, i, i, vp, pi])
     _sig(M, "mi_dpotrf_tiles_noalloc", i, [vp, i, i, vp, pi])
+    _sig(M, "mi_dpotrf_tiles_rec", i, [vp, i, i, vp, i, pi])
     _sig(M, "mi_dtd_create", i, [vp, i, i, vp, ctypes.POINTER(vp)])
     _sig(M, "mi_dtd_potrf", i, [vp, i])
     _sig(M, "mi_dtd_trsm", i, [vp, i, i])

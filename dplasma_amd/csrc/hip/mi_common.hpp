@@ -100,6 +100,7 @@ enum WsId {
     WS_ADJ_SEGS,        //   segment table
     WS_POTRI,           // inverse: the seed / result image
     WS_DTD,             // task-insertion front-end: per-task diagonal-block inverses
+    WS_POTRF_REC,       // recursive diagonal factor: packed image of one tile's sub-tiles
     WS_COUNT
 };
 struct DevBuf {

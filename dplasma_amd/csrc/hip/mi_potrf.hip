@@ -1128,6 +1128,29 @@ __global__ __launch_bounds__(256) void k_update_valu(double *__restrict__ T, int
         }
 }
 
+// Recursive diagonal factor (hmb): the lower triangle of one n x n tile
+// (column-major, ld) <-> a packed lower image of hmb x hmb sub-tiles (the
+// layout of the whole matrix, one level down).  One thread per element of
+// the image; the strict upper part of the diagonal sub-tiles and padding
+// rows are copied out as zeros and never copied back.
+template <class T, bool TO_IMG>
+__global__ __launch_bounds__(256) void k_subtile_copy(T *__restrict__ tile, int n, int ld, int hmb, int MTs,
+                                                      T *__restrict__ img, int64_t nel)
+{
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= nel) return;
+    const int64_t per = (int64_t)hmb * hmb;
+    int64_t t = e / per;
+    const int w = (int)(e - t * per), col = w / hmb, row = w - col * hmb;
+    int sn = 0;                                  // sub-tile column
+    while (t >= MTs - sn) { t -= MTs - sn; sn++; }
+    const int sm = sn + (int)t;
+    const int r = sm * hmb + row, cc = sn * hmb + col;
+    const bool in = r < n && cc < n && r >= cc;
+    if (TO_IMG) img[e] = in ? tile[(int64_t)cc * ld + r] : T(0);
+    else if (in) tile[(int64_t)cc * ld + r] = img[e];
+}
+
 __global__ void k_fill_int(int *p, int v, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1355,8 +1378,41 @@ constexpr size_t potrf_ws_elems(int NB) { return (size_t)2 * ((NB + 31) / 32) * 
 // ring: cross-stream dependencies use the context's fixed ring of events
 // (created with the context) instead of its growing pool; every event is
 // waited on within three records of being recorded, far inside the ring.
+// POTRF(k) of an n x n diagonal tile (ld NB) as a tile Cholesky of its own
+// with hmb x hmb sub-tiles (src/zpotrf_L.jdf:168-192: the RECURSIVE body
+// re-tiles the task with smallnb when the tile is larger), enqueued on s:
+// the tile is copied into a packed image of sub-tiles, factored there by
+// the same POTRF / TRSM / update launches (many workgroups for the panel
+// solves and updates inside the tile), and the factor copied back.  hmb is
+// a multiple of 32, so the 32 x 32 diagonal-block inverses of sub-step j
+// land in Winv at block j * hmb / 32, where the panel TRSM of the enclosing
+// step reads them.  img: the packed image, (MTs (MTs + 1) / 2) hmb^2
+// elements.
 template <class T>
-int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws, bool ring = false)
+int potrf_rec(Ctx *c, hipStream_t s, T *tile, int n, int NB, int hmb, T *Winv, int info_off, T *img)
+{
+    const int MTs = (n + hmb - 1) / hmb;
+    const int64_t nel = (int64_t)MTs * (MTs + 1) / 2 * hmb * hmb;
+    const unsigned grid = (unsigned)((nel + 255) / 256);
+    MI_LAUNCH(c, s, "potrf_rec_copy", 0.0, 16.0 * nel,
+              (k_subtile_copy<T, true><<<grid, 256, 0, s>>>(tile, n, NB, hmb, MTs, img, nel)));
+    int rc;
+    for (int j = 0; j < MTs; j++) {
+        T *w = Winv + (int64_t)(j * hmb / 32) * 1024;
+        if ((rc = launch_potrf<T>(c, s, img, n, hmb, MTs, j, w, info_off))) return rc;
+        if ((rc = launch_trsm<T>(c, s, img, n, hmb, MTs, j, w, -1, -1))) return rc;
+        if ((rc = launch_update<T>(c, s, img, n, hmb, MTs, j, j + 1, MTs, false, true))) return rc;
+    }
+    MI_LAUNCH(c, s, "potrf_rec_copy", 0.0, 16.0 * nel,
+              (k_subtile_copy<T, false><<<grid, 256, 0, s>>>(tile, n, NB, hmb, MTs, img, nel)));
+    return 0;
+}
+
+// hmb: recursive diagonal factor with sub-tiles of hmb (rounded down to a
+// multiple of 32, at least 32) for the diagonal tiles larger than that; 0:
+// one POTRF launch per step
+template <class T>
+int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws, bool ring = false, int hmb = 0)
 {
     *info = 0;
     if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
@@ -1369,8 +1425,20 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws,
     c->ev_next = 0;
     size_t ring_next = 0;
     auto ev = [&]() -> hipEvent_t { return ring ? c->ev_ring[ring_next++ % c->ev_ring.size()] : next_event(c); };
+    if (hmb > 0) hmb = std::max(32, hmb & ~31);
+    T *img = nullptr;
+    if (hmb > 0 && hmb < NB) {
+        const int MTs = (NB + hmb - 1) / hmb;
+        const size_t ib = (size_t)MTs * (MTs + 1) / 2 * hmb * hmb * sizeof(T);
+        if (!(img = (T *)ws_get(c, WS_POTRF_REC, ib))) return -3;
+    }
     auto diag = [&](int k) -> int {
-        return launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k, ws + (k & 1) * wsz);
+        T *w = ws + (k & 1) * wsz;
+        const int n = tile_rows(N, NB, MT, k);
+        if (img && n > hmb)
+            return potrf_rec<T>(c, c->s_panel, dT + tile_index(MT, k, k) * (int64_t)NB * NB, n, NB, hmb, w, k * NB,
+                                img);
+        return launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k, w);
     };
 
     MI_CHECK(hipEventRecord(c->t0, c->s_main));
@@ -1580,6 +1648,12 @@ int mi_spotrf_step_update(mi_ctx_t *c, int N, int NB, float *dT, int k, int n_be
 int mi_dpotrf_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
 {
     return potrf_tiles<double>(c, N, NB, dT, info, false, nullptr);
+}
+
+int mi_dpotrf_tiles_rec(mi_ctx_t *c, int N, int NB, double *dT, int hmb, int *info)
+{
+    if (hmb <= 0) { *info = 0; set_error("mi_dpotrf_tiles_rec: hmb=%d must be positive", hmb); return -2; }
+    return potrf_tiles<double>(c, N, NB, dT, info, false, nullptr, false, hmb);
 }
 
 int mi_dpotrf_tiles_noalloc(mi_ctx_t *c, int N, int NB, double *dT, int *info)

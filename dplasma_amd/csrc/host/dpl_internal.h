@@ -84,7 +84,7 @@ void dpl_forget_mat(const void *mat);
 int  dpl_is_sym_desc(const parsec_tiled_matrix_t *A);
 
 /* operations (dpl_potrf.c) */
-int dpl_run_potrf(parsec_context_t *ctx, int uplo, parsec_tiled_matrix_t *A, int *info);
+int dpl_run_potrf(parsec_context_t *ctx, int uplo, parsec_tiled_matrix_t *A, int hmb, int *info);
 int dpl_run_trsm(parsec_context_t *ctx, int side, int uplo, int trans, int diag, double alpha,
                  const parsec_tiled_matrix_t *A, parsec_tiled_matrix_t *B);
 int dpl_run_plgsy(parsec_context_t *ctx, double bump, int uplo, parsec_tiled_matrix_t *A,

@@ -15,7 +15,7 @@
 #include "dpl_internal.h"
 
 /* ----------------------------------------------------------- executors */
-int dpl_run_potrf(parsec_context_t *ctx, int uplo, parsec_tiled_matrix_t *A, int *info)
+int dpl_run_potrf(parsec_context_t *ctx, int uplo, parsec_tiled_matrix_t *A, int hmb, int *info)
 {
     const char *f = "dplasma_dpotrf";
     dpl_sym_dev_t s;
@@ -23,7 +23,8 @@ int dpl_run_potrf(parsec_context_t *ctx, int uplo, parsec_tiled_matrix_t *A, int
     int rc = dpl_sym_acquire(ctx, A, uplo, 1, &s, &ctx->last.stage_in_ms);
     if (rc) return rc;
     int iinfo = 0;
-    rc = mi_dpotrf_tiles(ctx->mi, s.N, s.NB, s.d, &iinfo);
+    rc = hmb > 0 ? mi_dpotrf_tiles_rec(ctx->mi, s.N, s.NB, s.d, hmb, &iinfo)
+                 : mi_dpotrf_tiles(ctx->mi, s.N, s.NB, s.d, &iinfo);
     if (rc) {
         dpl_sym_release(ctx, A, &s, 0, NULL);
         return dpl_mi_fail(f, rc);
@@ -92,8 +93,8 @@ void dplasma_dpotrf_Destruct(parsec_taskpool_t *tp) { free(tp); }
 
 void dplasma_dpotrf_setrecursive(parsec_taskpool_t *tp, int hmb)
 {
-    /* the diagonal tile is already split into 32-column panels inside one
-     * workgroup; the recursive tile size only has to be recorded */
+    /* diagonal tiles larger than hmb are factored as tile Choleskys of their
+     * own with hmb x hmb sub-tiles (mi_dpotrf_tiles_rec) */
     if (tp && hmb > 0) tp->hmb = hmb;
 }
 

@@ -93,7 +93,7 @@ int parsec_context_wait(parsec_context_t *ctx)
     for (int i = 0; i < ctx->npending; i++) {
         parsec_taskpool_t *tp = ctx->pending[i];
         switch (tp->kind) {
-        case DPL_TP_POTRF: tp->rc = dpl_run_potrf(ctx, tp->uplo, tp->A, tp->info); break;
+        case DPL_TP_POTRF: tp->rc = dpl_run_potrf(ctx, tp->uplo, tp->A, tp->hmb, tp->info); break;
         case DPL_TP_TRSM:
             tp->rc = dpl_run_trsm(ctx, tp->side, tp->uplo, tp->trans, tp->diag, tp->alpha, tp->Ac, tp->B);
             break;

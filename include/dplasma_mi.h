@@ -114,6 +114,13 @@ int mi_dpotrf_tiles(mi_ctx_t *ctx, int N, int NB, double *dT, int *info);
  * (v_mfma_f32_16x16x4_f32) trailing update, FP32 diagonal factor and panel
  * solve, same info rule.  Blocking. */
 int mi_spotrf_tiles(mi_ctx_t *ctx, int N, int NB, float *dT, int *info);
+/* mi_dpotrf_tiles with a recursive diagonal factor: POTRF(k) of every
+ * diagonal tile larger than hmb runs as a tile Cholesky of that tile with
+ * hmb x hmb sub-tiles (the RECURSIVE body of src/zpotrf_L.jdf:168-192 that
+ * dplasma_zpotrf_setrecursive / dplasma_zpotrf_rec select,
+ * src/zpotrf_wrapper.c:48-55,425-451).  hmb is rounded down to a multiple of
+ * 32 (at least 32); hmb >= NB changes nothing.  Same info.  Blocking. */
+int mi_dpotrf_tiles_rec(mi_ctx_t *ctx, int N, int NB, double *dT, int hmb, int *info);
 /* mi_dpotrf_tiles without any device allocation, for callers that cannot
  * allocate (code under stream capture, serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  Same arguments, results and
