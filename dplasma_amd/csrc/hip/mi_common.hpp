@@ -101,6 +101,7 @@ enum WsId {
     WS_POTRI,           // inverse: the seed / result image
     WS_DTD,             // task-insertion front-end: per-task diagonal-block inverses
     WS_POTRF_REC,       // recursive diagonal factor: packed image of one tile's sub-tiles
+    WS_BF16_PANEL,      // BF16-split FP32 factor: hi/lo images of one panel column
     WS_COUNT
 };
 struct DevBuf {

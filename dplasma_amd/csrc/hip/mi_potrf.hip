@@ -995,6 +995,174 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt
 }
 
 // ---------------------------------------------------------------------------
+// The BF16 two-term split of panel column k, once per step instead of once
+// per workgroup and K chunk of the update: tile (m, k), m >= m0, is written
+// to P as its hi and lo BF16 images transposed to [row][k] (NB x NB each,
+// row-major; rows past the tile's last row are zero), so that the update
+// stages runs of eight K values of one row with 16-byte loads straight into
+// LDS and does no conversion work.  One workgroup per 64 x 64 block of a
+// tile, transposed through LDS.
+constexpr int SPT = 64;
+__global__ __launch_bounds__(256) void k_split_panel(const float *__restrict__ Tt, int N, int NB, int MT, int k,
+                                                     int m0, __bf16 *__restrict__ P)
+{
+    __shared__ float sb[SPT][SPT + 1];
+    const int nb = (NB + SPT - 1) / SPT;
+    const int per = nb * nb;
+    const int t = blockIdx.x / per, s = blockIdx.x - t * per;
+    const int m = m0 + t;
+    const int bk = s / nb, br = s - bk * nb;       // block of K columns, block of rows
+    const int rows = tile_rows(N, NB, MT, m);
+    const float *A = Tt + tile_index(MT, m, k) * (int64_t)NB * NB;
+    __bf16 *H = P + (int64_t)t * 2 * NB * NB;
+    __bf16 *Lo = H + (int64_t)NB * NB;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < SPT * SPT; e += 256) {   // coalesced along the rows of a column
+        const int j = e / SPT, i = e % SPT;
+        const int kk = bk * SPT + j, r = br * SPT + i;
+        sb[j][i] = (kk < NB && r < rows) ? A[(int64_t)kk * NB + r] : 0.f;
+    }
+    __syncthreads();
+    for (int e = tid; e < SPT * SPT / 2; e += 256) {   // pairs of K, coalesced along a row
+        const int i = e / (SPT / 2), j = 2 * (e % (SPT / 2));
+        const int kk = bk * SPT + j, r = br * SPT + i;
+        if (r < NB && kk < NB) {                   // NB is even here (a multiple of 32)
+            const float x0 = sb[j][i], x1 = sb[j + 1][i];
+            const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1;
+            typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+            *(bf2 *)&H[(int64_t)r * NB + kk] = bf2{h0, h1};
+            *(bf2 *)&Lo[(int64_t)r * NB + kk] = bf2{(__bf16)(x0 - (float)h0), (__bf16)(x1 - (float)h1)};
+        }
+    }
+}
+
+// k_update_bf16x3 on the split panel P (tiles m0.. of column k, as written
+// by k_split_panel; NB a multiple of 32).  The A halves are negated on
+// their way into LDS (a sign-bit flip: the split of -a is minus the split
+// of a), so the MFMAs see exactly the operands of k_update_bf16x3 and the
+// result is the same bit for bit.  Staging: thread t holds rows t / 4 and
+// t / 4 + 64 at K offset 8 (t % 4) of the four images (A hi, A lo, B hi,
+// B lo).
+__global__ __launch_bounds__(256, 3) void k_update_bf16x3p(float *__restrict__ Tt, int N, int NB, int MT, int k,
+                                                          int n_begin, int m_off, int nsub,
+                                                          const __bf16 *__restrict__ P, int m0)
+{
+    __shared__ __align__(16) __bf16 Ah[BM * XS];
+    __shared__ __align__(16) __bf16 Al[BM * XS];
+    __shared__ __align__(16) __bf16 Bh[BM * XS];
+    __shared__ __align__(16) __bf16 Bl[BM * XS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int G8 = (G / 8) * 8;
+    const int L = (bid < G8) ? (bid % 8) * (G8 / 8) + bid / 8 : bid;
+    int m, n, bm, bn;
+    decode_block(L, nsub, n_begin, m_off, MT, &m, &n, &bm, &bn);
+    if (m == n && bm < bn) return;
+    const int rows_m = tile_rows(N, NB, MT, m);
+    const int cols_n = tile_rows(N, NB, MT, n);
+    const int r_base = bm * BM, c_base = bn * BM;
+    if (r_base >= rows_m || c_base >= cols_n) return;
+    const bool full = (r_base + BM <= rows_m) && (c_base + BM <= cols_n);
+    const int64_t img = (int64_t)NB * NB;
+    const __bf16 *AH = P + (int64_t)(m - m0) * 2 * img + (int64_t)r_base * NB;
+    const __bf16 *BH = P + (int64_t)(n - m0) * 2 * img + (int64_t)c_base * NB;
+    float *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
+    const int fr = lane & 15, fk = lane >> 4;
+
+    fx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
+    const bool pre = full && !((m == n) && (bm == bn));
+    if (pre) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = r_base + wm * 64 + i * 16 + fr;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    acc[i][j][q] = Cp[(int64_t)(c_base + wn * 64 + j * 16 + 4 * fk + q) * NB + row];
+            }
+    }
+
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    auto neg = [](bf8 v) -> bf8 { return __builtin_bit_cast(bf8, __builtin_bit_cast(u4, v) ^ 0x80008000u); };
+    const int rq = tid >> 2, kq = tid & 3;
+    bf8 sa[2][4];                                // [row half][A hi, A lo, B hi, B lo]
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int row = rq + 64 * i;
+            const int64_t o = (int64_t)row * NB + k0 + 8 * kq;
+            const bool ain = full || r_base + row < rows_m;
+            const bool bin = full || c_base + row < cols_n;
+            const bf8 z = bf8{};
+            sa[i][0] = ain ? *(const bf8 *)(AH + o) : z;
+            sa[i][1] = ain ? *(const bf8 *)(AH + img + o) : z;
+            sa[i][2] = bin ? *(const bf8 *)(BH + o) : z;
+            sa[i][3] = bin ? *(const bf8 *)(BH + img + o) : z;
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int o = (rq + 64 * i) * XS + 8 * kq;
+            *(bf8 *)&Ah[o] = neg(sa[i][0]);             // -A: the MFMA accumulates C - A B^T
+            *(bf8 *)&Al[o] = neg(sa[i][1]);
+            *(bf8 *)&Bh[o] = sa[i][2];
+            *(bf8 *)&Bl[o] = sa[i][3];
+        }
+    };
+
+    const int nstage = NB / XK;
+    gload(0);
+    for (int st = 0; st < nstage; st++) {
+        lstore();
+        __syncthreads();
+        if (st + 1 < nstage) gload((st + 1) * XK);
+        bf8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = wm * 64 + i * 16 + fr;
+            ah[i] = *(const bf8 *)&Ah[row * XS + 8 * fk];
+            al[i] = *(const bf8 *)&Al[row * XS + 8 * fk];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = wn * 64 + j * 16 + fr;
+            bh[j] = *(const bf8 *)&Bh[row * XS + 8 * fk];
+            bl[j] = *(const bf8 *)&Bl[row * XS + 8 * fk];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+            }
+        __syncthreads();
+    }
+
+    const bool diag = (m == n) && (bm == bn);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int rr = r_base + wm * 64 + i * 16 + fr;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int col = c_base + wn * 64 + j * 16 + 4 * fk + q;
+                if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
+                else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The same trailing update on the FP64 VALU (engine 0; the MFMA kernel
 // above is the default: with two resident waves per SIMD both reach about
 // the same FP64 rate on gfx950, and the MFMA kernel needs fewer registers
@@ -1169,6 +1337,10 @@ int g_update_waves = 0;
 // FP32 MFMA (default), 1 = BF16 MFMA, two-term split (the mixed solve passes
 // its choice as an argument instead)
 int g_supdate = 0;
+// BF16-split updates of the main stream: 1 = panel column split once per
+// step (k_split_panel + k_update_bf16x3p, default), 0 = split by every
+// workgroup (k_update_bf16x3)
+int g_bf16_panel_split = 1;
 
 // profiler names per precision
 template <class T> struct KN;
@@ -1187,6 +1359,7 @@ template <> struct KN<float> {
     static constexpr const char *update_la = "spotrf_update_la";
     static constexpr const char *update_bf16 = "spotrf_update_bf16x3";
     static constexpr const char *update_bf16_la = "spotrf_update_bf16x3_la";
+    static constexpr const char *split_panel = "spotrf_split_panel";
 };
 
 template <class T> bool valu_panel() { return std::is_same<T, double>::value && g_panel_kernel == 0; }
@@ -1261,7 +1434,8 @@ int launch_trsm(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, cons
 // through the last column).
 template <class T>
 int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin,
-                      int m_off, int64_t ntiles, bool bf16_split, bool la = false)
+                      int m_off, int64_t ntiles, bool bf16_split, bool la = false,
+                      const __bf16 *split = nullptr, int split_m0 = 0)
 {
     if (n_begin >= MT || n_begin + m_off >= MT) return 0;
     int64_t avail = MT - n_begin - m_off;
@@ -1305,6 +1479,10 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
         MI_LAUNCH(c, s, "dpotrf_update_valu", fl, by,
                   k_update_valu<<<(unsigned)grid, 256, 0, s>>>((double *)dT, N, NB, MT, k, n_begin, m_off,
                                                                nsub));
+    else if (bf16 && split)
+        MI_LAUNCH(c, s, la ? KN<float>::update_bf16_la : KN<float>::update_bf16, fl, by,
+                  k_update_bf16x3p<<<(unsigned)grid, 256, 0, s>>>((float *)dT, N, NB, MT, k, n_begin, m_off,
+                                                                  nsub, split, split_m0));
     else if (bf16)
         MI_LAUNCH(c, s, la ? KN<float>::update_bf16_la : KN<float>::update_bf16, fl, by,
                   k_update_bf16x3<<<(unsigned)grid, 256, 0, s>>>((float *)dT, N, NB, MT, k, n_begin, m_off,
@@ -1326,12 +1504,25 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
 
 template <class T>
 int launch_update(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, int n_begin, int n_end,
-                  bool bf16_split, bool la = false)
+                  bool bf16_split, bool la = false, const __bf16 *split = nullptr, int split_m0 = 0)
 {
     if (n_begin >= n_end) return 0;
     int64_t ntiles = 0;
     for (int n = n_begin; n < n_end; n++) ntiles += MT - n;
-    return launch_update_run<T>(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles, bf16_split, la);
+    return launch_update_run<T>(c, s, dT, N, NB, MT, k, n_begin, 0, ntiles, bf16_split, la, split, split_m0);
+}
+
+// BF16 split of the panel tiles (m, k), m >= m0, into P (k_split_panel)
+int launch_split_panel(Ctx *c, hipStream_t s, const float *dT, int N, int NB, int MT, int k, int m0, __bf16 *P)
+{
+    if (m0 >= MT) return 0;
+    const int nb = (NB + SPT - 1) / SPT;
+    const int64_t grid = (int64_t)(MT - m0) * nb * nb;
+    double by = 0.0;
+    for (int m = m0; m < MT; m++) by += (double)tile_rows(N, NB, MT, m) * NB * sizeof(float) + 2.0 * NB * NB * 2;
+    MI_LAUNCH(c, s, KN<float>::split_panel, 0.0, by,
+              k_split_panel<<<(unsigned)grid, 256, 0, s>>>(dT, N, NB, MT, k, m0, P));
+    return 0;
 }
 
 template <class T>
@@ -1432,6 +1623,14 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws,
         const size_t ib = (size_t)MTs * (MTs + 1) / 2 * hmb * hmb * sizeof(T);
         if (!(img = (T *)ws_get(c, WS_POTRF_REC, ib))) return -3;
     }
+    // FP32 factor with BF16 updates: the main stream splits each panel
+    // column once (tiles k+2.. of column k; its launches run in order, so one
+    // image serves every step)
+    __bf16 *split = nullptr;
+    if (std::is_same<T, float>::value && bf16_split && g_bf16_panel_split && NB % XK == 0 && MT > 2) {
+        split = (__bf16 *)ws_get(c, WS_BF16_PANEL, (size_t)(MT - 2) * 2 * NB * NB * sizeof(__bf16));
+        if (!split) return -3;
+    }
     auto diag = [&](int k) -> int {
         T *w = ws + (k & 1) * wsz;
         const int n = tile_rows(N, NB, MT, k);
@@ -1466,12 +1665,16 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws,
         // main stream: the look-ahead column k+2, then the bulk of step k
         MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
         col_ready = nullptr;
+        const __bf16 *sp = (split && k + 2 < MT) ? split : nullptr;
+        if (sp && (rc = launch_split_panel(c, c->s_main, (const float *)dT, N, NB, MT, k, k + 2, split))) return rc;
         if (k + 2 < MT) {
-            if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 2, k + 3, bf16_split))) return rc;
+            if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 2, k + 3, bf16_split, false, sp, k + 2)))
+                return rc;
             col_ready = ev();
             MI_CHECK(hipEventRecord(col_ready, c->s_main));
         }
-        if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 3, MT, bf16_split))) return rc;
+        if ((rc = launch_update<T>(c, c->s_main, dT, N, NB, MT, k, k + 3, MT, bf16_split, false, sp, k + 2)))
+            return rc;
         pk = pk_next;
     }
     MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
@@ -1605,6 +1808,13 @@ int mi_set_spotrf_update(int which)
 {
     if (which != 0 && which != 1) { set_error("FP32 update must be 0 (FP32 MFMA) or 1 (BF16 split)"); return -2; }
     g_supdate = which;
+    return 0;
+}
+
+int mi_set_bf16_panel_split(int which)
+{
+    if (which != 0 && which != 1) { set_error("BF16 panel split must be 0 (per workgroup) or 1 (once per step)"); return -2; }
+    g_bf16_panel_split = which;
     return 0;
 }
 

@@ -202,6 +202,12 @@ int mi_spotrf_step_update(mi_ctx_t *ctx, int N, int NB, float *dT, int k, int n_
  * MFMA with FP32 accumulation on a two-term bf16 split of both operands
  * (three products, ~16 significant bits).  Process-wide. */
 int mi_set_spotrf_update(int which);
+/* Where the BF16-split updates of an FP32 factor split their operands:
+ * 1 = each panel column once per step, into hi/lo BF16 images the main
+ * stream's updates load directly (default), 0 = every update workgroup
+ * splits the FP32 panel tiles it reads.  Both give the same factor bit for
+ * bit.  Process-wide. */
+int mi_set_bf16_panel_split(int which);
 
 /* ------------------------------------------------------------ task insertion */
 /* Task-insertion engine behind the DTD front-end of dplasma.h (the tile

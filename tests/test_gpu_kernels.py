@@ -401,3 +401,36 @@ def test_s_trailing_update_bf16x3(dev, po, N, NB):
             assert np.array_equal(tile(got, m, n)[tm:, :], C[tm:, :])
     assert worst < 4 * NB * 2.0 ** -16, worst
     assert worst > 0.0
+
+
+@pytest.mark.parametrize("N,NB", [(1536, 512), (1700, 320), (4096, 256), (2000, 128), (378, 93)])
+def test_bf16_panel_split_same_factor(dev, po, N, NB):
+    """The BF16-split FP32 factor whose main-stream updates load each panel
+    column split once per step into hi/lo BF16 images, against the factor
+    whose update workgroups split the FP32 panel tiles themselves: the same
+    bit for bit (the split of -a is minus the split of a, and rounding to
+    nearest is symmetric in sign).  NB = 93 is not a multiple of 32 and
+    takes the per-workgroup split both times."""
+    T = po.dplgsy_symtiles(N, NB, float(N), 3872).astype(np.float32)
+    out = []
+    dev.check(dev.M.mi_set_spotrf_update(1), "bf16x3")
+    try:
+        for which in (0, 1):
+            dev.check(dev.M.mi_set_bf16_panel_split(which), "panel split")
+            d = dev.upload(T)
+            info = ctypes.c_int(-1)
+            try:
+                dev.check(dev.M.mi_spotrf_tiles(dev.h, N, NB, d, ctypes.byref(info)), "spotrf")
+                out.append(dev.download(d, T.shape, np.float32))
+            finally:
+                dev.free(d)
+            assert info.value == 0
+    finally:
+        dev.M.mi_set_spotrf_update(0)
+        dev.M.mi_set_bf16_panel_split(1)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_bf16_panel_split_rejects_unknown(dev, dp):
+    assert dev.M.mi_set_bf16_panel_split(2) == -2
+    assert "panel split" in dp.last_error()
