@@ -997,11 +997,11 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt
 // ---------------------------------------------------------------------------
 // The BF16 two-term split of panel column k, once per step instead of once
 // per workgroup and K chunk of the update: tile (m, k), m >= m0, is written
-// to P as its hi and lo BF16 images transposed to [row][k] (NB x NB each,
-// row-major; rows past the tile's last row are zero), so that the update
-// stages runs of eight K values of one row with 16-byte loads straight into
-// LDS and does no conversion work.  One workgroup per 64 x 64 block of a
-// tile, transposed through LDS.
+// to P as its hi and lo BF16 images (NB x NB each; rows past the tile's
+// last row are zero) in chunk-major order [k / 32][row][k % 32], so that
+// the update's staging of one 32-wide K chunk of 128 rows reads 8 KB of
+// consecutive bytes (whole cache lines) and does no conversion work.  One
+// workgroup per 64 x 64 block of a tile, transposed through LDS.
 constexpr int SPT = 64;
 __global__ __launch_bounds__(256) void k_split_panel(const float *__restrict__ Tt, int N, int NB, int MT, int k,
                                                      int m0, __bf16 *__restrict__ P)
@@ -1030,8 +1030,9 @@ __global__ __launch_bounds__(256) void k_split_panel(const float *__restrict__ T
             const float x0 = sb[j][i], x1 = sb[j + 1][i];
             const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1;
             typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-            *(bf2 *)&H[(int64_t)r * NB + kk] = bf2{h0, h1};
-            *(bf2 *)&Lo[(int64_t)r * NB + kk] = bf2{(__bf16)(x0 - (float)h0), (__bf16)(x1 - (float)h1)};
+            const int64_t o = ((int64_t)(kk / XK) * NB + r) * XK + kk % XK;
+            *(bf2 *)&H[o] = bf2{h0, h1};
+            *(bf2 *)&Lo[o] = bf2{(__bf16)(x0 - (float)h0), (__bf16)(x1 - (float)h1)};
         }
     }
 }
@@ -1042,15 +1043,21 @@ __global__ __launch_bounds__(256) void k_split_panel(const float *__restrict__ T
 // of a), so the MFMAs see exactly the operands of k_update_bf16x3 and the
 // result is the same bit for bit.  Staging: thread t holds rows t / 4 and
 // t / 4 + 64 at K offset 8 (t % 4) of the four images (A hi, A lo, B hi,
-// B lo).
-__global__ __launch_bounds__(256, 3) void k_update_bf16x3p(float *__restrict__ Tt, int N, int NB, int MT, int k,
+// B lo), for two chunks ahead (two register sets: two waves per SIMD, the
+// loads of a chunk in flight over two chunks of MFMA work).  LDS rows are 64 bytes (four 16-byte chunks of eight K values),
+// chunk c of row r stored at position c ^ sw(r), sw(r) = ((r >> 1) & 3) ^
+// ((r >> 3) & 1): a fragment read (16 rows at one chunk) and a staging
+// store (4 rows x 4 chunks) then touch distinct banks in each pass, with
+// 32 banks (8 lanes a pass) and with 64 (16 lanes a pass).
+__global__ __launch_bounds__(256, 2) void k_update_bf16x3p(float *__restrict__ Tt, int N, int NB, int MT, int k,
                                                           int n_begin, int m_off, int nsub,
                                                           const __bf16 *__restrict__ P, int m0)
 {
-    __shared__ __align__(16) __bf16 Ah[BM * XS];
-    __shared__ __align__(16) __bf16 Al[BM * XS];
-    __shared__ __align__(16) __bf16 Bh[BM * XS];
-    __shared__ __align__(16) __bf16 Bl[BM * XS];
+    __shared__ __align__(16) __bf16 Ah[BM * XK];
+    __shared__ __align__(16) __bf16 Al[BM * XK];
+    __shared__ __align__(16) __bf16 Bh[BM * XK];
+    __shared__ __align__(16) __bf16 Bl[BM * XK];
+    auto lds = [](int r, int c) { return r * XK + 8 * (c ^ (((r >> 1) & 3) ^ ((r >> 3) & 1))); };
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int G = gridDim.x, bid = blockIdx.x;
@@ -1065,8 +1072,8 @@ __global__ __launch_bounds__(256, 3) void k_update_bf16x3p(float *__restrict__ T
     if (r_base >= rows_m || c_base >= cols_n) return;
     const bool full = (r_base + BM <= rows_m) && (c_base + BM <= cols_n);
     const int64_t img = (int64_t)NB * NB;
-    const __bf16 *AH = P + (int64_t)(m - m0) * 2 * img + (int64_t)r_base * NB;
-    const __bf16 *BH = P + (int64_t)(n - m0) * 2 * img + (int64_t)c_base * NB;
+    const __bf16 *AH = P + (int64_t)(m - m0) * 2 * img + (int64_t)r_base * XK;
+    const __bf16 *BH = P + (int64_t)(n - m0) * 2 * img + (int64_t)c_base * XK;
     float *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
     const int fr = lane & 15, fk = lane >> 4;
 
@@ -1091,50 +1098,45 @@ __global__ __launch_bounds__(256, 3) void k_update_bf16x3p(float *__restrict__ T
     typedef unsigned int u4 __attribute__((ext_vector_type(4)));
     auto neg = [](bf8 v) -> bf8 { return __builtin_bit_cast(bf8, __builtin_bit_cast(u4, v) ^ 0x80008000u); };
     const int rq = tid >> 2, kq = tid & 3;
-    bf8 sa[2][4];                                // [row half][A hi, A lo, B hi, B lo]
-    auto gload = [&](int k0) {
+    typedef bf8 Stage[2][4];                     // [row half][A hi, A lo, B hi, B lo]
+    Stage sa, sb;
+    auto gload = [&](Stage &x, int k0) {
 #pragma unroll
         for (int i = 0; i < 2; i++) {
             const int row = rq + 64 * i;
-            const int64_t o = (int64_t)row * NB + k0 + 8 * kq;
+            const int64_t o = ((int64_t)(k0 / XK) * NB + row) * XK + 8 * kq;
             const bool ain = full || r_base + row < rows_m;
             const bool bin = full || c_base + row < cols_n;
             const bf8 z = bf8{};
-            sa[i][0] = ain ? *(const bf8 *)(AH + o) : z;
-            sa[i][1] = ain ? *(const bf8 *)(AH + img + o) : z;
-            sa[i][2] = bin ? *(const bf8 *)(BH + o) : z;
-            sa[i][3] = bin ? *(const bf8 *)(BH + img + o) : z;
+            x[i][0] = ain ? *(const bf8 *)(AH + o) : z;
+            x[i][1] = ain ? *(const bf8 *)(AH + img + o) : z;
+            x[i][2] = bin ? *(const bf8 *)(BH + o) : z;
+            x[i][3] = bin ? *(const bf8 *)(BH + img + o) : z;
         }
     };
-    auto lstore = [&]() {
+    auto lstore = [&](const Stage &x) {
 #pragma unroll
         for (int i = 0; i < 2; i++) {
-            const int o = (rq + 64 * i) * XS + 8 * kq;
-            *(bf8 *)&Ah[o] = neg(sa[i][0]);             // -A: the MFMA accumulates C - A B^T
-            *(bf8 *)&Al[o] = neg(sa[i][1]);
-            *(bf8 *)&Bh[o] = sa[i][2];
-            *(bf8 *)&Bl[o] = sa[i][3];
+            const int o = lds(rq + 64 * i, kq);
+            *(bf8 *)&Ah[o] = neg(x[i][0]);             // -A: the MFMA accumulates C - A B^T
+            *(bf8 *)&Al[o] = neg(x[i][1]);
+            *(bf8 *)&Bh[o] = x[i][2];
+            *(bf8 *)&Bl[o] = x[i][3];
         }
     };
-
-    const int nstage = NB / XK;
-    gload(0);
-    for (int st = 0; st < nstage; st++) {
-        lstore();
-        __syncthreads();
-        if (st + 1 < nstage) gload((st + 1) * XK);
+    auto compute = [&]() {
         bf8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int row = wm * 64 + i * 16 + fr;
-            ah[i] = *(const bf8 *)&Ah[row * XS + 8 * fk];
-            al[i] = *(const bf8 *)&Al[row * XS + 8 * fk];
+            const int o = lds(wm * 64 + i * 16 + fr, fk);
+            ah[i] = *(const bf8 *)&Ah[o];
+            al[i] = *(const bf8 *)&Al[o];
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int row = wn * 64 + j * 16 + fr;
-            bh[j] = *(const bf8 *)&Bh[row * XS + 8 * fk];
-            bl[j] = *(const bf8 *)&Bl[row * XS + 8 * fk];
+            const int o = lds(wn * 64 + j * 16 + fr, fk);
+            bh[j] = *(const bf8 *)&Bh[o];
+            bl[j] = *(const bf8 *)&Bl[o];
         }
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -1144,7 +1146,26 @@ __global__ __launch_bounds__(256, 3) void k_update_bf16x3p(float *__restrict__ T
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
             }
+    };
+
+    // chunks in pairs: the loads of chunk c + 2 are issued while chunk c
+    // is computed (NB / 32 chunks; an odd count ends with a single one)
+    const int nstage = NB / XK;
+    gload(sa, 0);
+    if (nstage > 1) gload(sb, XK);
+    for (int st = 0; st < nstage; st += 2) {
+        lstore(sa);
         __syncthreads();
+        if (st + 2 < nstage) gload(sa, (st + 2) * XK);
+        compute();
+        __syncthreads();
+        if (st + 1 < nstage) {
+            lstore(sb);
+            __syncthreads();
+            if (st + 3 < nstage) gload(sb, (st + 3) * XK);
+            compute();
+            __syncthreads();
+        }
     }
 
     const bool diag = (m == n) && (bm == bn);
