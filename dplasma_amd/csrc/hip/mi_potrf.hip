@@ -1401,9 +1401,10 @@ int potrf_lds_bytes(int NB, int *RC)
 template <class T>
 int launch_potrf(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k, T *Winv, int info_off = 0)
 {
-    int RC;
-    int lds = potrf_lds_bytes<T>(NB, &RC);
+    // LDS and its row stride RC for the n rows of this tile (RC >= n + 1)
     const int n = tile_rows(N, NB, MT, k);
+    int RC;
+    int lds = potrf_lds_bytes<T>(n, &RC);
     T *A = dT + tile_index(MT, k, k) * (int64_t)NB * NB;
     const double fl = (double)n * n * n / 3.0;
     if (!valu_panel<T>()) {
@@ -1622,16 +1623,23 @@ int potrf_rec(Ctx *c, hipStream_t s, T *tile, int n, int NB, int hmb, T *Winv, i
 
 // hmb: recursive diagonal factor with sub-tiles of hmb (rounded down to a
 // multiple of 32, at least 32) for the diagonal tiles larger than that; 0:
-// one POTRF launch per step
+// one POTRF launch per step.  img_ext: the sub-tile image (nullptr: the
+// context's workspace).
+// Sub-tile size of the FP64 factor's diagonal tiles: for NB >= 512 (a
+// multiple of 64) the recursive diagonal factor with two sub-tiles of NB / 2
+// a side is faster (N = 32768, NB = 512: 202.2 against 204.2 ms), below that
+// the single POTRF launch.  Its image then has 3 (NB / 2)^2 elements.
+constexpr int dpotrf_default_hmb(int NB) { return (NB >= 512 && NB % 64 == 0) ? NB / 2 : 0; }
+
 template <class T>
-int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws, bool ring = false, int hmb = 0)
+int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws, bool ring = false, int hmb = 0,
+                T *img_ext = nullptr)
 {
     *info = 0;
     if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
     if (N == 0) return 0;
     const int MT = (N + NB - 1) / NB;
-    int rc = prepare_potrf<T>(NB);
-    if (rc) return rc;
+    int rc;
     const size_t wsz = potrf_ws_elems(NB) / 2;      // elements per step
     if (!ws && !(ws = (T *)ws_get(c, WS_POTRF_WINV, potrf_ws_elems(NB) * sizeof(T)))) return -3;
     c->ev_next = 0;
@@ -1642,8 +1650,12 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws,
     if (hmb > 0 && hmb < NB) {
         const int MTs = (NB + hmb - 1) / hmb;
         const size_t ib = (size_t)MTs * (MTs + 1) / 2 * hmb * hmb * sizeof(T);
-        if (!(img = (T *)ws_get(c, WS_POTRF_REC, ib))) return -3;
+        if (!(img = img_ext ? img_ext : (T *)ws_get(c, WS_POTRF_REC, ib))) return -3;
     }
+    // the largest diagonal block one POTRF launch factors: a sub-tile when
+    // the diagonal tiles are factored recursively (so NB may exceed what one
+    // launch's LDS holds), else a whole tile
+    if ((rc = prepare_potrf<T>(img ? hmb : NB))) return rc;
     // FP32 factor with BF16 updates: the main stream splits each panel
     // column once (tiles k+2.. of column k; its launches run in order, so one
     // image serves every step)
@@ -1782,16 +1794,20 @@ int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_sp
 // ---------------------------------------------------------------------------
 // Allocation-free FP64 factorization (mi_dpotrf_tiles_noalloc).  Its only
 // scratch memory, the 32 x 32 diagonal-block inverses that POTRF(k) hands to
-// TRSM(k) for two steps in flight, lives in this file-scope device array
-// instead of a context workspace: one copy per GPU, held until the process
-// ends, shared by every call on that GPU.  POTRF(k) writes each 1024-element
-// block whole before TRSM(k) reads it, so nothing has to be cleared between
-// calls.  On the host the array's name is only a stand-in: host code uses
+// TRSM(k) for two steps in flight and the sub-tile image of the recursive
+// diagonal factor (the same factor as mi_dpotrf_tiles), lives in this
+// file-scope device array instead of context workspaces: one copy per GPU,
+// held until the process ends, shared by every call on that GPU.  POTRF(k)
+// writes each 1024-element block whole before TRSM(k) reads it, and the
+// image is copied in from the tile before it is read, so nothing has to be
+// cleared between calls.  On the host the array's name is only a stand-in: host code uses
 // the device address that noalloc_setup reads once per device.
 namespace {
 constexpr int NOALLOC_MAX_NB = 1024;
 constexpr int NOALLOC_MAX_DEV = 64;
-__device__ __align__(256) double scratch[potrf_ws_elems(NOALLOC_MAX_NB)];   // 512 KiB
+constexpr size_t NOALLOC_WINV = potrf_ws_elems(NOALLOC_MAX_NB);                          // 512 KiB
+constexpr size_t NOALLOC_IMG = 3 * (size_t)(NOALLOC_MAX_NB / 2) * (NOALLOC_MAX_NB / 2);   // 6 MiB
+__device__ __align__(256) double scratch[NOALLOC_WINV + NOALLOC_IMG];
 
 std::mutex g_noalloc_mu;
 bool g_noalloc_done[NOALLOC_MAX_DEV];
@@ -1878,7 +1894,7 @@ int mi_spotrf_step_update(mi_ctx_t *c, int N, int NB, float *dT, int k, int n_be
 
 int mi_dpotrf_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
 {
-    return potrf_tiles<double>(c, N, NB, dT, info, false, nullptr);
+    return potrf_tiles<double>(c, N, NB, dT, info, false, nullptr, false, dpotrf_default_hmb(NB));
 }
 
 int mi_dpotrf_tiles_rec(mi_ctx_t *c, int N, int NB, double *dT, int hmb, int *info)
@@ -1915,7 +1931,8 @@ int mi_dpotrf_tiles_noalloc(mi_ctx_t *c, int N, int NB, double *dT, int *info)
     // this call
     const bool prof = c->prof;
     c->prof = false;
-    const int rc = potrf_tiles<double>(c, N, NB, dT, info, false, ptr, true);
+    const int hmb = dpotrf_default_hmb(NB);       // image: 3 (NB / 2)^2 <= NOALLOC_IMG elements
+    const int rc = potrf_tiles<double>(c, N, NB, dT, info, false, ptr, true, hmb, ptr + NOALLOC_WINV);
     c->prof = prof;
     return rc;
 }

@@ -106,8 +106,11 @@ int mi_dspd_from_orth_tiles(mi_ctx_t *ctx, int N, int NB, const double *dQ, doub
 /* ------------------------------------------------------------ hot path */
 /* In-place tile Cholesky, lower: the POTRF/TRSM/SYRK/GEMM task classes of
  * src/zpotrf_L.jdf:91-633 as fused per-step launches with one-panel
- * look-ahead.  *info receives k*NB + iinfo of the first non-positive pivot
- * (src/zpotrf_L.jdf:200-202), 0 on success.  Blocking. */
+ * look-ahead.  For NB >= 512 (a multiple of 64) the diagonal tiles are
+ * factored recursively with sub-tiles of NB / 2 (as mi_dpotrf_tiles_rec
+ * with hmb = NB / 2), which is faster there.  *info receives k*NB + iinfo of
+ * the first non-positive pivot (src/zpotrf_L.jdf:200-202), 0 on success.
+ * Blocking. */
 int mi_dpotrf_tiles(mi_ctx_t *ctx, int N, int NB, double *dT, int *info);
 /* Single precision of the same factorization on FP32 tiles (the "s"
  * precision of src/zpotrf_wrapper.c:345-371): FP32 MFMA
@@ -124,12 +127,13 @@ int mi_dpotrf_tiles_rec(mi_ctx_t *ctx, int N, int NB, double *dT, int hmb, int *
 /* mi_dpotrf_tiles without any device allocation, for callers that cannot
  * allocate (code under stream capture, serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  Same arguments, results and
- * info as mi_dpotrf_tiles (bitwise the same factor); POTRF(k) runs as one
- * launch per step.  Its scratch (the diagonal-block inverses of two steps in
- * flight) is a fixed 512 KiB device array, one per GPU, so NB <= 1024
- * (larger NB: -2 and an error message).  Calls that overlap on one GPU race
- * on that array, whichever context they use: serialize them.  The context's
- * profiler is not used by this call.  Blocking. */
+ * info as mi_dpotrf_tiles (bitwise the same factor, including its recursive
+ * diagonal factor for NB >= 512).  Its scratch (the diagonal-block inverses
+ * of two steps in flight and one diagonal tile's sub-tile image) is a fixed
+ * 6.5 MiB device array, one per GPU, so NB <= 1024 (larger NB: -2 and an
+ * error message).  Calls that overlap on one GPU race on that array,
+ * whichever context they use: serialize them.  The context's profiler is not
+ * used by this call.  Blocking. */
 int mi_dpotrf_tiles_noalloc(mi_ctx_t *ctx, int N, int NB, double *dT, int *info);
 /* Left/Lower tile triangular solve, B <- alpha op(L)^{-1} B, op = NoTrans
  * (111) or Trans (112), diag NonUnit (131) / Unit (132):

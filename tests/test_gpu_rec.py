@@ -4,8 +4,10 @@ hmb is factored as a tile Cholesky of its own with hmb x hmb sub-tiles.
 Against the oracle's tile-DAG factor of the same matrix (the order of the
 operations inside a diagonal tile differs, so the bound is the relative 1e-12
 of the other factor tests), on ragged tilings, an hmb that is not a multiple
-of 32 (rounded down), hmb >= NB (no recursion), the info of a non-SPD matrix,
-and the descriptor path with the reference's residual check, both uplo."""
+of 32 (rounded down), hmb >= NB (no recursion), NB = 1024 (whose whole-tile
+POTRF would not fit the LDS, so only the recursive factor runs it), the info
+of a non-SPD matrix, and the descriptor path with the reference's residual
+check, both uplo."""
 import ctypes
 
 import numpy as np
@@ -26,7 +28,7 @@ def factor_rec(dev, T, N, NB, hmb):
 
 
 @pytest.mark.parametrize("N,NB,hmb", [(4096, 512, 128), (1700, 320, 64), (2048, 512, 100), (378, 93, 32),
-                                      (1000, 256, 32), (600, 512, 128), (2000, 256, 512)])
+                                      (1000, 256, 32), (600, 512, 128), (2000, 256, 512), (3000, 1024, 512)])
 def test_rec_matches_oracle(dev, po, N, NB, hmb):
     print(f"dpotrf_rec N={N} NB={NB} hmb={hmb}", flush=True)
     T = po.dplgsy_symtiles(N, NB, float(N), 3872)
@@ -45,6 +47,26 @@ def test_rec_matches_oracle(dev, po, N, NB, hmb):
         t0 = T[idx * NB * NB:(idx + 1) * NB * NB].reshape(NB, NB)
         up = np.triu(np.ones((NB, NB), dtype=bool), 1).T              # row < col
         assert np.array_equal(tg[up], t0[up])
+
+
+@pytest.mark.parametrize("N,NB", [(4096, 512), (3000, 1024), (2000, 256)])
+def test_default_diagonal_factor(dev, po, N, NB):
+    """mi_dpotrf_tiles factors the diagonal tiles recursively with sub-tiles
+    of NB / 2 for NB >= 512 (a multiple of 64) and with one POTRF launch
+    below: bitwise the factor of mi_dpotrf_tiles_rec with hmb = NB / 2, or
+    with hmb >= NB (no recursion)."""
+    T = po.dplgsy_symtiles(N, NB, float(N), 3872)
+    d = dev.upload(T)
+    info = ctypes.c_int(-1)
+    try:
+        dev.check(dev.M.mi_dpotrf_tiles(dev.h, N, NB, d, ctypes.byref(info)), "dpotrf")
+        a = dev.download(d, T.shape)
+    finally:
+        dev.free(d)
+    assert info.value == 0
+    b, ib = factor_rec(dev, T, N, NB, NB // 2 if NB >= 512 else NB)
+    assert ib == 0
+    assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("r", [0, 700, 1999])

@@ -1,7 +1,9 @@
 """mi_dpotrf_tiles_noalloc (the factorization that allocates nothing, its
 scratch in a fixed per-GPU device array) against mi_dpotrf_tiles on the same
-tiles: bitwise the same factor and info, on two calls in a row (the second
-smaller, with a ragged last tile), plus the rejection of NB beyond the fixed
+tiles: bitwise the same factor and info, on calls in a row (the first at the
+largest NB its scratch holds, whose recursive diagonal factor fills the whole
+sub-tile image; then smaller, the last with a ragged last tile and the
+single-launch diagonal factor), plus the rejection of NB beyond the fixed
 scratch and a non-SPD matrix's info."""
 import ctypes
 
@@ -23,7 +25,7 @@ def run(dev, entry, T, N, NB):
 
 
 def test_noalloc_matches_dpotrf_tiles_twice(dev, po):
-    for N, NB in [(4096, 512), (1700, 320)]:      # second call smaller, ragged
+    for N, NB in [(4096, 1024), (4096, 512), (1700, 320)]:   # smaller each call, the last ragged
         print(f"noalloc: N={N} NB={NB}", flush=True)
         T = po.dplgsy_symtiles(N, NB, float(N), 3872)
         a, ia = run(dev, "mi_dpotrf_tiles", T, N, NB)
