@@ -1624,16 +1624,20 @@ int potrf_rec(Ctx *c, hipStream_t s, T *tile, int n, int NB, int hmb, T *Winv, i
 // hmb: recursive diagonal factor with sub-tiles of hmb (rounded down to a
 // multiple of 32, at least 32) for the diagonal tiles larger than that; 0:
 // one POTRF launch per step.  img_ext: the sub-tile image (nullptr: the
-// context's workspace).
-// Sub-tile size of the FP64 factor's diagonal tiles: for NB >= 512 (a
-// multiple of 64) the recursive diagonal factor with two sub-tiles of NB / 2
-// a side is faster (N = 32768, NB = 512: 202.2 against 204.2 ms), below that
-// the single POTRF launch.  Its image then has 3 (NB / 2)^2 elements.
-constexpr int dpotrf_default_hmb(int NB) { return (NB >= 512 && NB % 64 == 0) ? NB / 2 : 0; }
+// context's workspace).  rec_from: the first step whose diagonal tile is
+// factored recursively (the earlier ones take one POTRF launch).
+// Sub-tile size of the recursive diagonal factor used by default: two
+// sub-tiles of NB / 2 a side for NB >= 512 (a multiple of 64), none below
+// (the image then has 3 (NB / 2)^2 elements).  The FP64 factor uses it for
+// every step (N = 32768, NB = 512: 202.2 against 204.2 ms); the BF16-split
+// FP32 factor only from step MT / 2 on, where the panel chain is critical
+// (59.6 against 61.6-61.9 ms; from step 0 60.7 ms: in the first half the
+// extra panel launches compete with the bulk update).
+constexpr int half_tile_hmb(int NB) { return (NB >= 512 && NB % 64 == 0) ? NB / 2 : 0; }
 
 template <class T>
 int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws, bool ring = false, int hmb = 0,
-                T *img_ext = nullptr)
+                T *img_ext = nullptr, int rec_from = 0)
 {
     *info = 0;
     if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
@@ -1667,7 +1671,7 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws,
     auto diag = [&](int k) -> int {
         T *w = ws + (k & 1) * wsz;
         const int n = tile_rows(N, NB, MT, k);
-        if (img && n > hmb)
+        if (img && n > hmb && k >= rec_from)
             return potrf_rec<T>(c, c->s_panel, dT + tile_index(MT, k, k) * (int64_t)NB * NB, n, NB, hmb, w, k * NB,
                                 img);
         return launch_potrf<T>(c, c->s_panel, dT, N, NB, MT, k, w);
@@ -1787,7 +1791,9 @@ size_t task_winv_elems(int NB) { return potrf_ws_elems(NB) / 2; }
 
 int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split)
 {
-    return potrf_tiles<float>(c, N, NB, dT, info, bf16_split, nullptr);
+    const int MT = NB > 0 ? (N + NB - 1) / NB : 0;
+    return potrf_tiles<float>(c, N, NB, dT, info, bf16_split, nullptr, false, bf16_split ? half_tile_hmb(NB) : 0,
+                              nullptr, MT / 2);
 }
 }  // namespace mi
 
@@ -1894,7 +1900,7 @@ int mi_spotrf_step_update(mi_ctx_t *c, int N, int NB, float *dT, int k, int n_be
 
 int mi_dpotrf_tiles(mi_ctx_t *c, int N, int NB, double *dT, int *info)
 {
-    return potrf_tiles<double>(c, N, NB, dT, info, false, nullptr, false, dpotrf_default_hmb(NB));
+    return potrf_tiles<double>(c, N, NB, dT, info, false, nullptr, false, half_tile_hmb(NB));
 }
 
 int mi_dpotrf_tiles_rec(mi_ctx_t *c, int N, int NB, double *dT, int hmb, int *info)
@@ -1931,7 +1937,7 @@ int mi_dpotrf_tiles_noalloc(mi_ctx_t *c, int N, int NB, double *dT, int *info)
     // this call
     const bool prof = c->prof;
     c->prof = false;
-    const int hmb = dpotrf_default_hmb(NB);       // image: 3 (NB / 2)^2 <= NOALLOC_IMG elements
+    const int hmb = half_tile_hmb(NB);            // image: 3 (NB / 2)^2 <= NOALLOC_IMG elements
     const int rc = potrf_tiles<double>(c, N, NB, dT, info, false, ptr, true, hmb, ptr + NOALLOC_WINV);
     c->prof = prof;
     return rc;
@@ -1939,7 +1945,7 @@ int mi_dpotrf_tiles_noalloc(mi_ctx_t *c, int N, int NB, double *dT, int *info)
 
 int mi_spotrf_tiles(mi_ctx_t *c, int N, int NB, float *dT, int *info)
 {
-    return potrf_tiles<float>(c, N, NB, dT, info, g_supdate == 1, nullptr);
+    return spotrf_tiles_split(c, N, NB, dT, info, g_supdate == 1);
 }
 
 }  // extern "C"

@@ -431,6 +431,35 @@ def test_bf16_panel_split_same_factor(dev, po, N, NB):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("N,NB", [(2048, 512), (4096, 512), (3000, 1024), (1700, 320)])
+def test_spotrf_bf16x3_factor(dev, po, N, NB):
+    """The BF16-split FP32 factor (whose diagonal tiles from step MT / 2 on
+    are factored recursively with sub-tiles of NB / 2 when NB >= 512)
+    against the FP64 oracle factor of the same FP32 input: the split keeps
+    ~16 bits of each operand and each of the MT steps adds one update, so
+    the factor is good to MT 2^-16 relative (a wrong sub-tile or image
+    mapping gives an O(1) error)."""
+    T = po.dplgsy_symtiles(N, NB, float(N), 3872).astype(np.float32)
+    dev.check(dev.M.mi_set_spotrf_update(1), "bf16x3")
+    d = dev.upload(T)
+    info = ctypes.c_int(-1)
+    try:
+        dev.check(dev.M.mi_spotrf_tiles(dev.h, N, NB, d, ctypes.byref(info)), "spotrf")
+        got = dev.download(d, T.shape, np.float32)
+    finally:
+        dev.free(d)
+        dev.M.mi_set_spotrf_update(0)
+    assert info.value == 0
+    ref = T.astype(np.float64)
+    assert po.dpotrf_tiles(ref, N, NB) == 0
+    L = np.tril(po.symtiles_to_full(got.astype(np.float64), N, NB))
+    Lo = np.tril(po.symtiles_to_full(ref, N, NB))
+    rel = np.abs(L - Lo).max() / np.abs(Lo).max()
+    print(f"bf16x3 factor N={N} NB={NB}: rel {rel:.3e}", flush=True)
+    MT = (N + NB - 1) // NB
+    assert rel <= MT * 2.0 ** -16, rel
+
+
 def test_bf16_panel_split_rejects_unknown(dev, dp):
     assert dev.M.mi_set_bf16_panel_split(2) == -2
     assert "panel split" in dp.last_error()
