@@ -36,19 +36,20 @@ using namespace mi;
 
 namespace {
 
-constexpr int BW = 32;            // panel width
+constexpr int BW = 32;            // diagonal block / panel-solve width
+constexpr int PW = 64;            // width of the panel the long update covers
 constexpr int BMAX = 1024;        // largest matrix order of a batch
 
-template <class T, int SH>
+template <class T, int SH, int BWC>
 struct BLds {
     static constexpr int LPA = SH + 128 / (int)sizeof(T);
-    static constexpr int LPB = BW + 128 / (int)sizeof(T);
+    static constexpr int LPB = BWC + 128 / (int)sizeof(T);
 };
 
 template <class T>
 struct BShared {
     T sa[BW * (128 + 128 / sizeof(T))];   // -L rows of the strip, [kk][row]
-    T sb[BW * (BW + 128 / sizeof(T))];    // L rows of the diagonal block, [kk][col]
+    T sb[BW * (PW + 128 / sizeof(T))];    // L rows of the panel's columns, [kk][col]
     T ws[BW][BW + 1];                     // its inverse
     T ys[BMAX];                           // solve vector
     T red[8][BW];
@@ -56,31 +57,40 @@ struct BShared {
     int fail;
 };
 
-// 1. C(0:M, 0:jb) -= L(j0:n, 0:j0) L(j0:j0+jb, 0:j0)^T, C = A(j0:n, j0:j0+jb)
-template <class T, int SH>
-__device__ void panel_update(T *__restrict__ A, int n, int j0, int jb, BShared<T> &sh)
+// 1. C(0:M, 0:jb) -= L(j0:n, k0:k1) L(j0:j0+jb, k0:k1)^T, C = A(j0:n, j0:j0+jb),
+// jb <= BWC (k0, k1 multiples of 32): the long update of a panel of BWC
+// columns over every column left of it (k0 = 0, k1 = j0), or the update of
+// the second half of a 64-wide panel by its first half
+template <class T, int SH, int BWC>
+__device__ void panel_update(T *__restrict__ A, int n, int j0, int jb, int k0, int k1, BShared<T> &sh)
 {
     typedef typename MF<T>::v4 V4;
-    constexpr int LPA = BLds<T, SH>::LPA, LPB = BLds<T, SH>::LPB;
+    constexpr int LPA = BLds<T, SH, BWC>::LPA, LPB = BLds<T, SH, BWC>::LPB;
+    constexpr int TC = BWC / 16;                // 16-column MFMA tiles
+    constexpr int NB_ = BWC * BW / 256;         // staged B elements per thread
     constexpr int TW = SH / 64;                 // 16-row MFMA tiles per wave
     constexpr int NA = SH * BW / 256;           // staged A elements per thread
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // opaque per call: offsets derived from tid are recomputed here instead
+    // of being held live across the caller's loops
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
     const int M = n - j0;
     const int lda = n;   // n <= 1024: 32-bit offsets
     for (int s = 0; s < M; s += SH) {
-        V4 acc[TW][2];
+        V4 acc[TW][TC];
 #pragma unroll
         for (int i = 0; i < TW; i++)
 #pragma unroll
-            for (int jj = 0; jj < 2; jj++)
+            for (int jj = 0; jj < TC; jj++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int row = s + wave * (SH / 4) + i * 16 + fr;
                     const int col = jj * 16 + MF<T>::row(q, fk);
                     acc[i][jj][q] = (row < M && col < jb && row >= col) ? A[(j0 + col) * lda + j0 + row] : T(0);
                 }
-        T ra[NA], rb[4];
+        T ra[NA], rb[NB_];
         auto gload = [&](int kc) {
 #pragma unroll
             for (int p = 0; p < NA; p++) {
@@ -88,8 +98,8 @@ __device__ void panel_update(T *__restrict__ A, int n, int j0, int jb, BShared<T
                 ra[p] = (s + r < M) ? -A[(kc + kk) * lda + j0 + s + r] : T(0);
             }
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-                const int e = tid + 256 * p, kk = e / BW, c = e - kk * BW;
+            for (int p = 0; p < NB_; p++) {
+                const int e = tid + 256 * p, kk = e / BWC, c = e - kk * BWC;
                 rb[p] = (c < jb) ? A[(kc + kk) * lda + j0 + c] : T(0);
             }
         };
@@ -100,35 +110,35 @@ __device__ void panel_update(T *__restrict__ A, int n, int j0, int jb, BShared<T
                 sh.sa[kk * LPA + r] = ra[p];
             }
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-                const int e = tid + 256 * p, kk = e / BW, c = e - kk * BW;
+            for (int p = 0; p < NB_; p++) {
+                const int e = tid + 256 * p, kk = e / BWC, c = e - kk * BWC;
                 sh.sb[kk * LPB + c] = rb[p];
             }
         };
-        if (j0 > 0) gload(0);
-        for (int kc = 0; kc < j0; kc += BW) {
+        if (k0 < k1) gload(k0);
+        for (int kc = k0; kc < k1; kc += BW) {
             __syncthreads();                    // previous chunk consumed
             lstore();
             __syncthreads();
-            if (kc + BW < j0) gload(kc + BW);
+            if (kc + BW < k1) gload(kc + BW);
 #pragma unroll
             for (int kq = 0; kq < BW / 4; kq++) {
                 const int kk = kq * 4 + fk;
-                T a[TW], b[2];
+                T a[TW], b[TC];
 #pragma unroll
                 for (int i = 0; i < TW; i++) a[i] = sh.sa[kk * LPA + wave * (SH / 4) + i * 16 + fr];
 #pragma unroll
-                for (int jj = 0; jj < 2; jj++) b[jj] = sh.sb[kk * LPB + jj * 16 + fr];
+                for (int jj = 0; jj < TC; jj++) b[jj] = sh.sb[kk * LPB + jj * 16 + fr];
 #pragma unroll
                 for (int i = 0; i < TW; i++)
 #pragma unroll
-                    for (int jj = 0; jj < 2; jj++) acc[i][jj] = MF<T>::mfma(b[jj], a[i], acc[i][jj]);
+                    for (int jj = 0; jj < TC; jj++) acc[i][jj] = MF<T>::mfma(b[jj], a[i], acc[i][jj]);
             }
         }
 #pragma unroll
         for (int i = 0; i < TW; i++)
 #pragma unroll
-            for (int jj = 0; jj < 2; jj++)
+            for (int jj = 0; jj < TC; jj++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int row = s + wave * (SH / 4) + i * 16 + fr;
@@ -204,7 +214,11 @@ __device__ void panel_solve(T *__restrict__ A, int n, int j0, int jb, BShared<T>
 {
     typedef typename MF<T>::v4 V4;
     constexpr int TW = SH / 64;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // opaque per call: offsets derived from tid are recomputed here instead
+    // of being held live across the caller's loops
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6;
     const int fr = lane & 15, fk = lane >> 4;
     const int lda = n;   // n <= 1024: 32-bit offsets
     for (int s = j0 + jb; s < n; s += SH) {
@@ -244,19 +258,32 @@ __device__ void panel_solve(T *__restrict__ A, int n, int j0, int jb, BShared<T>
     }
 }
 
+// Left-looking over panels of PW = 64 columns: one long update of the whole
+// panel by every column left of it (the strip of L to the left is streamed
+// once per 64 columns instead of once per 32), then the panel's two halves
+// of BW = 32: diagonal block, the rows below it, and the update of the
+// second half by the first.
 template <class T>
 __device__ int factor_one(T *__restrict__ A, int n, bool narrow, BShared<T> &sh)
 {
-    for (int j0 = 0; j0 < n; j0 += BW) {
-        const int jb = min(BW, n - j0);
-        if (narrow) panel_update<T, 64>(A, n, j0, jb, sh);
-        else panel_update<T, 128>(A, n, j0, jb, sh);
+    for (int p0 = 0; p0 < n; p0 += PW) {
+        const int pb = min(PW, n - p0);
+        if (narrow) panel_update<T, 64, PW>(A, n, p0, pb, 0, p0, sh);
+        else panel_update<T, 128, PW>(A, n, p0, pb, 0, p0, sh);
         __syncthreads();
-        const int f = diag_block<T>(A, n, j0, jb, sh);
-        if (f) return j0 + f;
-        if (narrow) panel_solve<T, 64>(A, n, j0, jb, sh);
-        else panel_solve<T, 128>(A, n, j0, jb, sh);
-        __syncthreads();
+        for (int j0 = p0; j0 < p0 + pb; j0 += BW) {
+            const int jb = min(BW, n - j0);
+            if (j0 > p0) {
+                if (narrow) panel_update<T, 64, BW>(A, n, j0, jb, p0, j0, sh);
+                else panel_update<T, 128, BW>(A, n, j0, jb, p0, j0, sh);
+                __syncthreads();
+            }
+            const int f = diag_block<T>(A, n, j0, jb, sh);
+            if (f) return j0 + f;
+            if (narrow) panel_solve<T, 64>(A, n, j0, jb, sh);
+            else panel_solve<T, 128>(A, n, j0, jb, sh);
+            __syncthreads();
+        }
     }
     return 0;
 }
