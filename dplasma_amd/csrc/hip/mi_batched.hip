@@ -37,7 +37,10 @@ using namespace mi;
 namespace {
 
 constexpr int BW = 32;            // diagonal block / panel-solve width
-constexpr int PW = 64;            // width of the panel the long update covers
+// width of the panel the long update covers: 64 in FP64 (two workgroups of
+// 76 KB LDS per CU), 128 in FP32 (42 -> 50 KB; in FP64 the 92 KB it needs
+// would leave one workgroup per CU: 36.3 against 28.8 ms for the batch)
+template <class T> constexpr int panel_width() { return sizeof(T) == 8 ? 64 : 128; }
 constexpr int BMAX = 1024;        // largest matrix order of a batch
 
 template <class T, int SH, int BWC>
@@ -49,7 +52,7 @@ struct BLds {
 template <class T>
 struct BShared {
     T sa[BW * (128 + 128 / sizeof(T))];   // -L rows of the strip, [kk][row]
-    T sb[BW * (PW + 128 / sizeof(T))];    // L rows of the panel's columns, [kk][col]
+    T sb[BW * (panel_width<T>() + 128 / sizeof(T))];   // L rows of the panel's columns, [kk][col]
     T ws[BW][BW + 1];                     // its inverse
     T ys[BMAX];                           // solve vector
     T red[8][BW];
@@ -258,14 +261,15 @@ __device__ void panel_solve(T *__restrict__ A, int n, int j0, int jb, BShared<T>
     }
 }
 
-// Left-looking over panels of PW = 64 columns: one long update of the whole
-// panel by every column left of it (the strip of L to the left is streamed
-// once per 64 columns instead of once per 32), then the panel's two halves
-// of BW = 32: diagonal block, the rows below it, and the update of the
-// second half by the first.
+// Left-looking over panels of PW = 64 (FP64) or 128 (FP32) columns: one long
+// update of the whole panel by every column left of it (the strip of L to
+// the left is streamed once per PW columns instead of once per 32), then
+// the panel's blocks of BW = 32: the update by the panel's earlier blocks,
+// diagonal block, and the rows below it.
 template <class T>
 __device__ int factor_one(T *__restrict__ A, int n, bool narrow, BShared<T> &sh)
 {
+    constexpr int PW = panel_width<T>();
     for (int p0 = 0; p0 < n; p0 += PW) {
         const int pb = min(PW, n - p0);
         if (narrow) panel_update<T, 64, PW>(A, n, p0, pb, 0, p0, sh);
