@@ -526,11 +526,14 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="one untimed factorization, for rocprofv3 runs")
     ap.add_argument("--profile-kind", default="dpotrf",
-                    choices=["dpotrf", "spotrf", "spotrf_bf16x3", "dpotrs", "adjoint", "batched", "batched_fp32"],
+                    choices=["dpotrf", "spotrf", "spotrf_bf16x3", "dpotrs", "dsposv_bf16x3", "adjoint", "batched",
+                             "batched_fp32"],
                     help="what --profile-only runs: the FP64 factorization, the FP32 factorization (FP32 "
                          "MFMA or BF16-split updates) of the same matrix rounded, the FP64 solve (16 RHS), "
-                         "the adjoint of the FP64 factorization, or one launch of the batched solve of "
-                         "4096 uneven matrices (FP64 or FP32) at the tuned shape")
+                         "the mixed-precision solve with the BF16-split factor (kappa = 1e2, 16 RHS; two "
+                         "calls, the first sizes the workspaces), the adjoint of the FP64 factorization, or "
+                         "one launch of the batched solve of 4096 uneven matrices (FP64 or FP32) at the tuned "
+                         "shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the plain copy kernel on 1 GiB (scales HBM counters)")
     args = ap.parse_args()
@@ -586,6 +589,27 @@ def main():
             else:
                 dev.check(mi.mi_dposv_batched(p, A, B, 0), "dposv_batched")
             mi.mi_batch_destroy(p)
+        print("profile step done")
+        return
+    if args.profile_only and args.profile_kind == "dsposv_bf16x3":
+        import ctypes
+        mi, _ = dp.libs()
+        with dp.Device(0) as dev:
+            MT = (N + NB - 1) // NB
+            nt = MT * (MT + 1) // 2 * NB * NB
+            dQ = dev.alloc(N * N * 8)
+            dev.check(mi.mi_dorth_haar(dev.h, N, 7, dQ), "orth")
+            dA, dW = dev.alloc(nt * 8), dev.alloc(nt * 8)
+            dev.check(mi.mi_dspd_from_orth_tiles(dev.h, N, NB, dQ, 1e2, dA), "spd")
+            dev.free(dQ)
+            dB, dX = dev.alloc(N * 16 * 8), dev.alloc(N * 16 * 8)
+            dev.check(mi.mi_dplrnt(dev.h, N, 16, dB, N, N, 0, 0, 3873), "B")
+            info, it = ctypes.c_int(), ctypes.c_int()
+            for _ in range(2):
+                dev.check(mi.mi_memcpy_d2d(dev.h, dW, dA, nt * 8), "copy")
+                dev.check(mi.mi_dsposv_tiles(dev.h, N, NB, dW, 16, dB, N, dX, N, 1, ctypes.byref(it),
+                                             ctypes.byref(info)), "dsposv")
+                assert info.value == 0 and it.value > 0, (info.value, it.value)
         print("profile step done")
         return
     if args.profile_only and args.profile_kind == "adjoint":

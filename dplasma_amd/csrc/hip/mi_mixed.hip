@@ -58,25 +58,40 @@ __global__ __launch_bounds__(256) void k_d2s(const double *__restrict__ in, floa
 // 0 on the strict upper parts of the diagonal tiles and on padding rows, and
 // *ovf = 1 when a value of the lower triangle exceeds FLT_MAX in magnitude
 // (dlat2s semantics: only the referenced triangle is converted and tested).
+// Workgroup b converts part b % per of packed tile b / per (tiles are stored
+// column by column, so b / per is the tile's storage index); with NB a
+// multiple of 4 each thread moves runs of four elements of one column.
 __global__ __launch_bounds__(256) void k_d2s_lower(const double *__restrict__ S, float *__restrict__ D, int N,
-                                                  int NB, int MT, int64_t n, int *__restrict__ ovf)
+                                                  int NB, int MT, int per, int *__restrict__ ovf)
 {
+    int t = blockIdx.x / per;
+    const int part = blockIdx.x - t * per;
+    const int64_t base = (int64_t)t * NB * NB;
+    int col = 0;                                   // tile column of tile t
+    while (t >= MT - col) { t -= MT - col; col++; }
+    const int row = col + t;
+    const int rows = min(NB, N - row * NB), cols = min(NB, N - col * NB);
+    const bool dg = row == col;
+    const int T2 = NB * NB;
+    auto conv = [&](double x, int r, int cl, bool &bad) {
+        const bool ref = r < rows && cl < cols && (!dg || r >= cl);
+        bad |= ref && fabs(x) > (double)FLT_MAX;
+        return ref ? (float)x : 0.f;
+    };
     bool bad = false;
-    const int64_t T2 = (int64_t)NB * NB;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t t = i / T2;
-        const int e = (int)(i - t * T2), r = e % NB, cl = e / NB;
-        int col = 0;                       // tile column of tile t
-        while (t >= MT - col) { t -= MT - col; col++; }
-        const int row = col + (int)t;
-        const bool ref = row * NB + r < N && col * NB + cl < N && (row != col || r >= cl);
-        float v = 0.f;
-        if (ref) {
-            const double x = S[i];
-            bad |= fabs(x) > (double)FLT_MAX;
-            v = (float)x;
+    if ((NB & 3) == 0) {
+        for (int q = part * 256 + (int)threadIdx.x; q < T2 / 4; q += per * 256) {
+            const int e = 4 * q, cl = e / NB, r = e - cl * NB;
+            const double2 x0 = *(const double2 *)(S + base + e), x1 = *(const double2 *)(S + base + e + 2);
+            const float4 v = {conv(x0.x, r, cl, bad), conv(x0.y, r + 1, cl, bad), conv(x1.x, r + 2, cl, bad),
+                              conv(x1.y, r + 3, cl, bad)};
+            *(float4 *)(D + base + e) = v;
         }
-        D[i] = v;
+    } else {
+        for (int e = part * 256 + (int)threadIdx.x; e < T2; e += per * 256) {
+            const int cl = e / NB;
+            D[base + e] = conv(S[base + e], e - cl * NB, cl, bad);
+        }
     }
     if (bad) *ovf = 1;
 }
@@ -590,8 +605,14 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
     const auto t_start = std::chrono::steady_clock::now();
     // A -> FP32 tiles (the lower triangle; the rest of the image is zeroed)
     k_flag_clear<<<1, 1, 0, s>>>(c->d_info);
-    MI_LAUNCH(c, s, "dlag2s", 0.0, 12.0 * (double)ntile_el,
-              k_d2s_lower<<<grid_of(ntile_el), 256, 0, s>>>(dA, L32, N, NB, MT, ntile_el, c->d_info));
+    {
+        // four runs of four elements per thread and tile part
+        const int per = (int)std::max<int64_t>(1, ((int64_t)NB * NB + 4095) / 4096);
+        const int64_t nblk = (int64_t)MT * (MT + 1) / 2 * per;
+        if (nblk > 0x7fffffff) { set_error("dlag2s grid too large"); rc = -3; goto done; }
+        MI_LAUNCH(c, s, "dlag2s", 0.0, 12.0 * (double)ntile_el,
+                  k_d2s_lower<<<(unsigned)nblk, 256, 0, s>>>(dA, L32, N, NB, MT, per, c->d_info));
+    }
     MX_CHECK(hipMemcpyAsync(&ovf, c->d_info, sizeof(int), hipMemcpyDeviceToHost, s), done);
     MX_CHECK(hipStreamSynchronize(s), done);
     if (ovf) { rc = fall_back(-2); goto done; }

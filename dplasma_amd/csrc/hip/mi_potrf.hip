@@ -1183,6 +1183,119 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3p(float *__restrict__ T
     }
 }
 
+// k_update_bf16x3p with 256 x 256 blocks (NB a multiple of 256): half the
+// panel operand traffic per flop.  Eight waves as a 2 x 4 grid of 128 x 64;
+// the four images' K chunks go straight into LDS by global_load_lds_dwordx4
+// (no staging registers), double-buffered in 128 KB (one workgroup per CU).
+// A direct load cannot negate, so the B fragments are negated in registers
+// instead: (-b) a is the same exact BF16 product as b (-a), so the MFMAs
+// sum the same values and the result is that of k_update_bf16x3p bit for
+// bit.  Lane l of a wave's load of 16 rows lands at LDS offset 16 l (row
+// 16 g + l / 4, slot l % 4), so it fetches K chunk (l % 4) ^ sw(row) of
+// that row.  Rows up to r_base + 255 lie inside the NB-row tile image,
+// zero past the tile's last row, so no load needs a guard.
+typedef __attribute__((address_space(3))) void *lds_ptr;
+constexpr int WB = 256;
+__global__ __launch_bounds__(512, 1) void k_update_bf16x3w(float *__restrict__ Tt, int N, int NB, int MT, int k,
+                                                          int n_begin, int m_off, int nsub,
+                                                          const __bf16 *__restrict__ P, int m0)
+{
+    __shared__ __align__(16) __bf16 S[2][4][WB * XK];     // [buffer][A hi, A lo, B hi, B lo]
+    auto sw = [](int r) { return ((r >> 1) & 3) ^ ((r >> 3) & 1); };
+    auto lds = [&](int r, int c) { return r * XK + 8 * (c ^ sw(r)); };
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 2, wn = wave & 3;
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int G8 = (G / 8) * 8;
+    const int L = (bid < G8) ? (bid % 8) * (G8 / 8) + bid / 8 : bid;
+    int m, n, bm, bn;
+    decode_block(L, nsub, n_begin, m_off, MT, &m, &n, &bm, &bn);
+    if (m == n && bm < bn) return;
+    const int rows_m = tile_rows(N, NB, MT, m);
+    const int cols_n = tile_rows(N, NB, MT, n);
+    const int r_base = bm * WB, c_base = bn * WB;
+    if (r_base >= rows_m || c_base >= cols_n) return;
+    const int64_t img = (int64_t)NB * NB;
+    const __bf16 *AH = P + (int64_t)(m - m0) * 2 * img + (int64_t)r_base * XK;
+    const __bf16 *BH = P + (int64_t)(n - m0) * 2 * img + (int64_t)c_base * XK;
+    float *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
+    const int fr = lane & 15, fk = lane >> 4;
+
+    // Each wave's 128 x 64 lies inside one 128 x 128 block (pr, pc) of
+    // k_update_bf16x3p, and takes the path that kernel takes there (C as
+    // the MFMA's start, or C += the products), so that both round alike.
+    const int pr = r_base + wm * 128, pc = c_base + (wn >> 1) * 128;
+    const bool pre = pr + 128 <= rows_m && pc + 128 <= cols_n && !(m == n && pr <= pc);
+    fx4 acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                acc[i][j][q] = pre ? Cp[(int64_t)(c_base + wn * 64 + j * 16 + 4 * fk + q) * NB + pr + i * 16 + fr]
+                                   : 0.f;
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    auto neg = [](bf8 v) -> bf8 { return __builtin_bit_cast(bf8, __builtin_bit_cast(u4, v) ^ 0x80008000u); };
+    // wave w fetches rows 32 w .. 32 w + 31 of the four images
+    auto dload = [&](int buf, int k0) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int R = wave * 32 + 16 * h, lr = R + (lane >> 2);
+            const int lc = (lane & 3) ^ sw(lr);
+            const int64_t o = ((int64_t)(k0 / XK) * NB + lr) * XK + 8 * lc;
+            __builtin_amdgcn_global_load_lds(AH + o, (lds_ptr)&S[buf][0][R * XK], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(AH + img + o, (lds_ptr)&S[buf][1][R * XK], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(BH + o, (lds_ptr)&S[buf][2][R * XK], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(BH + img + o, (lds_ptr)&S[buf][3][R * XK], 16, 0, 0);
+        }
+    };
+    const int nstage = NB / XK;
+    dload(0, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int st = 0; st < nstage; st++) {
+        const int b = st & 1;
+        if (st + 1 < nstage) dload(b ^ 1, (st + 1) * XK);   // into the buffer read at step st - 1
+        bf8 bh[4], bl[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int o = lds(wn * 64 + j * 16 + fr, fk);
+            bh[j] = neg(*(const bf8 *)&S[b][2][o]);          // -B: the MFMA accumulates C - A B^T
+            bl[j] = neg(*(const bf8 *)&S[b][3][o]);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int o = lds(wm * 128 + i * 16 + fr, fk);
+            const bf8 ah = *(const bf8 *)&S[b][0][o], al = *(const bf8 *)&S[b][1][o];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
+            }
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");      // this wave's loads of step st + 1
+        __syncthreads();                                      // everyone's, and buffer b read
+    }
+
+    const bool diag = m == n;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int rr = pr + i * 16 + fr;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int col = c_base + wn * 64 + j * 16 + 4 * fk + q;
+                if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
+                else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
+            }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The same trailing update on the FP64 VALU (engine 0; the MFMA kernel
 // above is the default: with two resident waves per SIMD both reach about
@@ -1501,6 +1614,12 @@ int launch_update_run(Ctx *c, hipStream_t s, T *dT, int N, int NB, int MT, int k
         MI_LAUNCH(c, s, "dpotrf_update_valu", fl, by,
                   k_update_valu<<<(unsigned)grid, 256, 0, s>>>((double *)dT, N, NB, MT, k, n_begin, m_off,
                                                                nsub));
+    else if (bf16 && split && NB % WB == 0 && ntiles * (NB / WB) * (NB / WB) >= 2 * 256)
+        // 256 x 256 blocks once the launch fills the chip twice over (one
+        // workgroup per CU)
+        MI_LAUNCH(c, s, la ? KN<float>::update_bf16_la : KN<float>::update_bf16, fl, by,
+                  k_update_bf16x3w<<<(unsigned)(ntiles * (NB / WB) * (NB / WB)), 512, 0, s>>>(
+                      (float *)dT, N, NB, MT, k, n_begin, m_off, NB / WB, split, split_m0));
     else if (bf16 && split)
         MI_LAUNCH(c, s, la ? KN<float>::update_bf16_la : KN<float>::update_bf16, fl, by,
                   k_update_bf16x3p<<<(unsigned)grid, 256, 0, s>>>((float *)dT, N, NB, MT, k, n_begin, m_off,

@@ -624,6 +624,19 @@ template <bool TRANS, int TPW> __device__ inline int wv_row(int t, int i)
 }
 template <bool TRANS> __device__ inline int wv_k(int kq, int f) { return TRANS ? 16 * (kq >> 2) + 4 * f + (kq & 3) : 4 * kq + f; }
 
+// Four consecutive elements from p (16-byte aligned): one 16-byte load in
+// FP32, two in FP64.
+template <class T> __device__ inline void wv_load4(const T *p, T (&v)[4])
+{
+    if constexpr (sizeof(T) == 4) {
+        const fx4 w = *(const fx4 *)p;
+        v[0] = w[0]; v[1] = w[1]; v[2] = w[2]; v[3] = w[3];
+    } else {
+        const double2 a = *(const double2 *)p, b = *(const double2 *)(p + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+}
+
 // Staging position of X(row R of the block, right-hand side j).  FP64: [row]
 // [rhs], read one element per lane and load.  FP32: consumer order, two
 // values per 64-bit load: the element kq of lane (j, f) of slice hb sits at
@@ -703,50 +716,63 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
     // a-operands of one other block h: op(L)(my row wv_row(t, fr), column
     // 32 hb + wv_k(kq, fk) of the block)
     T la[4][8][TPW];
-    auto load_a = [&](int h) {
+    // (the addresses are clamped into the matrix).  FP64 zeroes what lies
+    // outside the matrix as the values arrive; FP32, whose loads run two
+    // blocks ahead, leaves that to mask_a, once the values are needed.
+    constexpr bool MASK_AT_LOAD = sizeof(T) == 8;
+    auto load_a = [&](T (&la)[4][8][TPW], int h) {
 #pragma unroll
         for (int hb = 0; hb < 4; hb++) {
             const int c0 = h * WV_RB + 32 * hb;
             const int cc = min(c0, N - 1) & ~31;     // a valid 32-aligned block when c0 >= N
             const int rr = min(rw, N - 1) & ~(RW - 1);
+            auto put = [&](int kq, int t, T v) {
+                if (MASK_AT_LOAD) {
+                    const int kk = wv_k<TRANS>(kq, fk), row = rw + wv_row<TRANS, TPW>(t, fr);
+                    v = (c0 < N && c0 + kk < N && row < N) ? v : T(0);
+                }
+                la[hb][kq][t] = v;
+            };
             if (!TRANS) {
                 if constexpr (TPW == 2) {
                     const T *pA = sym_elem(Tt, MT, NB, rr, cc) + 2 * fr;
 #pragma unroll
                     for (int kq = 0; kq < 8; kq++) {
-                        const int kk = 4 * kq + fk;
-                        const V2 v = *(const V2 *)(pA + (int64_t)kk * NB);
-                        const bool ck = c0 < N && c0 + kk < N;
-                        la[hb][kq][0] = (ck && rw + 2 * fr < N) ? v.x : T(0);
-                        la[hb][kq][1] = (ck && rw + 2 * fr + 1 < N) ? v.y : T(0);
+                        const V2 v = *(const V2 *)(pA + (int64_t)(4 * kq + fk) * NB);
+                        put(kq, 0, v.x);
+                        put(kq, 1, v.y);
                     }
                 } else {
                     const T *pA = sym_elem(Tt, MT, NB, rr, cc) + fr;
 #pragma unroll
-                    for (int kq = 0; kq < 8; kq++) {
-                        const int kk = 4 * kq + fk;
-                        const T v = pA[(int64_t)kk * NB];
-                        la[hb][kq][0] = (c0 < N && c0 + kk < N && rw + fr < N) ? v : T(0);
-                    }
+                    for (int kq = 0; kq < 8; kq++) put(kq, 0, pA[(int64_t)(4 * kq + fk) * NB]);
                 }
             } else {
                 const T *pA = sym_elem(Tt, MT, NB, cc, rr) + 4 * fk;
 #pragma unroll
-                for (int t = 0; t < TPW; t++) {
-                    const int i = 16 * t + fr;
+                for (int t = 0; t < TPW; t++)
 #pragma unroll
                     for (int kg = 0; kg < 2; kg++) {
-                        const T *p = pA + (int64_t)i * NB + 16 * kg;
-                        const V2 v0 = *(const V2 *)p, v1 = *(const V2 *)(p + 2);
-                        const T v[4] = {v0.x, v0.y, v1.x, v1.y};
+                        T v[4];
+                        wv_load4(pA + (int64_t)(16 * t + fr) * NB + 16 * kg, v);
 #pragma unroll
-                        for (int e = 0; e < 4; e++) {
-                            const int kk = 16 * kg + 4 * fk + e;
-                            la[hb][4 * kg + e][t] = (c0 < N && rw + i < N && c0 + kk < N) ? v[e] : T(0);
-                        }
+                        for (int e = 0; e < 4; e++) put(4 * kg + e, t, v[e]);
                     }
-                }
             }
+        }
+    };
+    auto mask_a = [&](T (&la)[4][8][TPW], int h) {
+        if ((h + 1) * WV_RB <= N && rw + RW <= N) return;     // wave-uniform
+#pragma unroll
+        for (int hb = 0; hb < 4; hb++) {
+            const int c0 = h * WV_RB + 32 * hb;
+#pragma unroll
+            for (int kq = 0; kq < 8; kq++)
+#pragma unroll
+                for (int t = 0; t < TPW; t++) {
+                    const int kk = wv_k<TRANS>(kq, fk), row = rw + wv_row<TRANS, TPW>(t, fr);
+                    if (!(c0 < N && c0 + kk < N && row < N)) la[hb][kq][t] = T(0);
+                }
         }
     };
     // a-operands of the link matrix: M_g(RW w + wv_row(t, fr), 32 hb + wv_k(kq, fk)),
@@ -771,12 +797,10 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
                 for (int t = 0; t < TPW; t++)
 #pragma unroll
                     for (int kg = 0; kg < 2; kg++) {
-                        const T *p = Mg + (RW * wave + 16 * t + fr) * 128 + 32 * hb + 16 * kg + 4 * fk;
-                        const V2 v0 = *(const V2 *)p, v1 = *(const V2 *)(p + 2);
-                        la[hb][4 * kg + 0][t] = v0.x;
-                        la[hb][4 * kg + 1][t] = v0.y;
-                        la[hb][4 * kg + 2][t] = v1.x;
-                        la[hb][4 * kg + 3][t] = v1.y;
+                        T v[4];
+                        wv_load4(Mg + (RW * wave + 16 * t + fr) * 128 + 32 * hb + 16 * kg + 4 * fk, v);
+#pragma unroll
+                        for (int e = 0; e < 4; e++) la[hb][4 * kg + e][t] = v[e];
                     }
             }
         }
@@ -788,12 +812,15 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
     // true once this wait has to end without its values: the abort word is
     // set, or this wait outlived its budget (then it sets the word)
     int *abort_word = ticket + 1;
+    // (global-memory pointers: a generic one would make every later wait
+    // for a vector load a wait for all of them)
+    typedef __attribute__((address_space(1))) int GI;
     auto give_up = [&](long long t_end) -> bool {
-        const int *ap = abort_word;
+        GI *ap = (GI *)abort_word;
         asm volatile("" : "+v"(ap));
         if (__hip_atomic_load(ap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return true;
         if (wait_ticks > 0 && (long long)__builtin_amdgcn_s_memrealtime() < t_end) return false;
-        int *wp = abort_word;
+        GI *wp = (GI *)abort_word;
         asm volatile("" : "+v"(wp));
         __hip_atomic_store(wp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return true;
@@ -846,19 +873,103 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
             }
     };
     const int nprev = TRANS ? nwg - 1 - g : g;
-    if (nprev > 1 && rw < N) load_a(TRANS ? nwg - 1 : 0);
-    for (int s = 0; s + 1 < nprev; s++) {
-        const int h = TRANS ? nwg - 1 - s : s;
-        if (rw < N) {
-            T bx[4][8];
-            wait_x(h, false, bx);
+    auto blk = [&](int s) { return TRANS ? nwg - 1 - s : s; };   // the s-th earlier block
+    auto mfma_block = [&](const T (&l)[4][8][TPW], const T (&bx)[4][8]) {
+#pragma unroll
+        for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+            for (int kq = 0; kq < 8; kq++)
+#pragma unroll
+                for (int t = 0; t < TPW; t++) acc[t] = MF<T>::mfma(l[hb][kq][t], bx[hb][kq], acc[t]);
+    };
+    if constexpr (sizeof(T) == 4) {
+        // Every earlier block but the last is long finished by the time it
+        // is needed, so its X is read ahead like its L: the L operands and
+        // one read of X of block s + 1 are issued before block s is waited
+        // for and multiplied, two blocks in flight.  They are issued after
+        // the loads of block s and vector-memory loads complete in order, so
+        // waiting for block s leaves them outstanding.  The loads are
+        // unconditional (the last block is read once more at the end) so
+        // that no register holding a load in flight is copied.  A read-ahead
+        // X that still holds a sentinel is read again by the bounded wait.
+        typedef unsigned long long X2[4][4];
+        typedef __attribute__((address_space(1))) const unsigned long long GU;
+        T lb[4][8][TPW];
+        X2 xa, xb;
+        auto read_x = [&](int h, X2 &xr) {
+            const T *xh = xs + (int64_t)h * (WV_RB * RHSB);
 #pragma unroll
             for (int hb = 0; hb < 4; hb++)
 #pragma unroll
-                for (int kq = 0; kq < 8; kq++)
+                for (int kp = 0; kp < 4; kp++) {
+                    GU *p = (GU *)(xh + ((hb * 4 + kp) * 64 + lane) * 2);
+                    xr[hb][kp] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+        };
+        auto ready = [&](const X2 &xr) {
+            int ok = 1;
 #pragma unroll
-                    for (int t = 0; t < TPW; t++) acc[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], acc[t]);
-            if (s + 2 < nprev) load_a(TRANS ? nwg - 2 - s : s + 1);
+            for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+                for (int kp = 0; kp < 4; kp++)
+                    ok &= ((unsigned)xr[hb][kp] != (unsigned)WvSent<float>::v) &
+                          ((unsigned)(xr[hb][kp] >> 32) != (unsigned)WvSent<float>::v);
+            return __all(ok);
+        };
+        auto consume = [&](int s, T (&lc)[4][8][TPW], X2 &xc) {
+            if (!ready(xc)) {
+                // the bounded wait of wait_x, reading into the same registers
+                const long long t_end = (long long)__builtin_amdgcn_s_memrealtime() + wait_ticks;
+                if (wait_ticks == 0) (void)give_up(t_end);
+                for (;;) {
+                    if (__any(give_up(t_end))) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    read_x(blk(s), xc);
+                    if (ready(xc)) break;
+                }
+            }
+            T bx[4][8];
+#pragma unroll
+            for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+                for (int kp = 0; kp < 4; kp++) {
+                    bx[hb][2 * kp] = __builtin_bit_cast(T, (unsigned)xc[hb][kp]);
+                    bx[hb][2 * kp + 1] = __builtin_bit_cast(T, (unsigned)(xc[hb][kp] >> 32));
+                }
+#pragma unroll
+            for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+                for (int kq = 0; kq < 8; kq++) {
+                    const int r = blk(s) * WV_RB + 32 * hb + wv_k<TRANS>(kq, fk);
+                    bx[hb][kq] = (fr < nr && r < N) ? -bx[hb][kq] : T(0);
+                }
+            mask_a(lc, blk(s));
+            mfma_block(lc, bx);
+        };
+        if (nprev > 1 && rw < N) {
+            const int last = nprev - 2;           // the last block of this loop
+            load_a(la, blk(0));
+            read_x(blk(0), xa);
+            for (int s = 0;;) {
+                load_a(lb, blk(min(s + 1, last)));
+                read_x(blk(min(s + 1, last)), xb);
+                consume(s, la, xa);
+                if (++s > last) break;
+                load_a(la, blk(min(s + 1, last)));
+                read_x(blk(min(s + 1, last)), xa);
+                consume(s, lb, xb);
+                if (++s > last) break;
+            }
+        }
+    } else {
+        if (nprev > 1 && rw < N) load_a(la, blk(0));
+        for (int s = 0; s + 1 < nprev; s++) {
+            if (rw < N) {
+                T bx[4][8];
+                wait_x(blk(s), false, bx);
+                mfma_block(la, bx);
+                if (s + 2 < nprev) load_a(la, blk(s + 1));
+            }
         }
     }
     if (nprev > 0 && rw < N) load_m();

@@ -274,9 +274,14 @@ def test_spotrf_info(dev, po):
 @pytest.mark.parametrize("N,NB,nrhs,trans", [(700, 128, 1, 111), (700, 128, 1, 112), (1000, 320, 16, 111),
                                              (1000, 320, 16, 112), (378, 93, 5, 112),
                                              (3000, 256, 17, 111), (3000, 256, 17, 112),
-                                             (1000, 64, 16, 112), (1000, 96, 3, 111), (1000, 96, 3, 112)])
+                                             (1000, 64, 16, 112), (1000, 96, 3, 111), (1000, 96, 3, 112),
+                                             (5000, 256, 16, 111), (5000, 256, 16, 112),
+                                             (4999, 512, 2, 111), (4999, 512, 2, 112)])
 def test_strsm_LL_kernel(dev, po, N, NB, nrhs, trans):
-    """FP32 tile solve against the FP64 oracle on the same FP32 data."""
+    """FP32 tile solve against the FP64 oracle on the same FP32 data.  With
+    NB a multiple of 32 the wavefront kernel runs; at N = 5000 and 4999 its
+    workgroups multiply up to 38 earlier blocks of 128 rows two at a time
+    (an even and an odd count, the last block 8 or 7 rows)."""
     L, T = _tri_tiles(po, N, NB, N + nrhs)
     T32 = T.astype(np.float32)
     rng = np.random.default_rng(9)
@@ -403,14 +408,17 @@ def test_s_trailing_update_bf16x3(dev, po, N, NB):
     assert worst > 0.0
 
 
-@pytest.mark.parametrize("N,NB", [(1536, 512), (1700, 320), (4096, 256), (2000, 128), (378, 93)])
+@pytest.mark.parametrize("N,NB", [(1536, 512), (1700, 320), (4096, 256), (2000, 128), (378, 93),
+                                  (9512, 512), (12000, 256)])
 def test_bf16_panel_split_same_factor(dev, po, N, NB):
     """The BF16-split FP32 factor whose main-stream updates load each panel
     column split once per step into hi/lo BF16 images, against the factor
     whose update workgroups split the FP32 panel tiles themselves: the same
     bit for bit (the split of -a is minus the split of a, and rounding to
     nearest is symmetric in sign).  NB = 93 is not a multiple of 32 and
-    takes the per-workgroup split both times."""
+    takes the per-workgroup split both times.  At (9512, 512) and
+    (12000, 256) the early main-stream launches cover enough tiles for the
+    256 x 256 block kernel (ragged last tiles of 296 and 224 rows)."""
     T = po.dplgsy_symtiles(N, NB, float(N), 3872).astype(np.float32)
     out = []
     dev.check(dev.M.mi_set_spotrf_update(1), "bf16x3")

@@ -55,8 +55,10 @@ def test_dsposv_overflow_in_rhs_gives_iter_minus2(dev, po):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_dsposv_ignores_unreferenced_triangle(dev, po, prec):
-    N, NB, nrhs = 300, 64, 3
+@pytest.mark.parametrize("NB", [64, 93])
+def test_dsposv_ignores_unreferenced_triangle(dev, po, prec, NB):
+    """NB = 64 converts runs of four elements, NB = 93 one at a time."""
+    N, nrhs = 300, 3
     A = spd(N, 3)
     T = po.full_to_symtiles(A, NB)
     MT = (N + NB - 1) // NB
@@ -68,6 +70,21 @@ def test_dsposv_ignores_unreferenced_triangle(dev, po, prec):
     B = np.random.default_rng(1).uniform(-0.5, 0.5, (N, nrhs))
     X, it, info = run_dsposv(dev, T, N, NB, B, prec)
     assert info == 0 and 0 <= it <= 30, it
+    assert np.abs(X - np.linalg.solve(A, B)).max() < 1e-10
+
+
+@pytest.mark.parametrize("NB", [64, 93])
+def test_dsposv_overflow_in_matrix_gives_iter_minus2(dev, po, NB):
+    """A diagonal entry above FLT_MAX in the referenced triangle (the matrix
+    stays SPD): the rounding to FP32 overflows, ITER = -2, and the FP64
+    factor and solve give the answer."""
+    N, nrhs = 300, 2
+    A = spd(N, 5)
+    A[200, 200] = 1e39
+    T = po.full_to_symtiles(A, NB)
+    B = np.random.default_rng(2).uniform(-0.5, 0.5, (N, nrhs))
+    X, it, info = run_dsposv(dev, T, N, NB, B, 1)
+    assert it == -2 and info == 0
     assert np.abs(X - np.linalg.solve(A, B)).max() < 1e-10
 
 
