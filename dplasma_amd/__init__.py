@@ -170,6 +170,7 @@ def libs():
     _sig(M, "mi_set_panel_engine", i, [i])
     _sig(M, "mi_set_spotrf_update", i, [i])
     _sig(M, "mi_set_bf16_panel_split", i, [i])
+    _sig(M, "mi_set_symm_kernel", i, [i])
     _sig(M, "mi_set_solve_engine", i, [i])
     _sig(M, "mi_set_solve_wait_limit", i, [d])
     _sig(M, "mi_dcheck_potrf", i, [vp, i, i, vp, vp, pd, pd])

@@ -428,6 +428,260 @@ __global__ __launch_bounds__(256, 2) void k_symm_part(const double *__restrict__
     }
 }
 
+// The same partials for NB = 128 RT (RT = 1..4), the tile crossing HBM once
+// as whole 16-column chunks (NB x 16 contiguous doubles) loaded straight
+// into LDS by global_load_lds_dwordx4, double-buffered, so that the next
+// chunk is in flight while this one is used.  One workgroup of eight waves
+// per stored tile (m, n); wave w owns rows [16 RT w, 16 RT (w + 1)):
+//   P1 rows of its slice:   A(rows, chunk) X_n(chunk)       (RT x 4 MFMAs)
+//   P2 of the chunk's 16 columns over its rows:
+//                           A(rows, chunk)^T X_m(rows)      (4 RT MFMAs)
+// Both read their a-operands from the same LDS copy (P2's transposed).  The
+// eight waves' P2 contributions are summed in LDS in a fixed order, so the
+// result is deterministic.  Masks (diagonal tile, ragged tiles) are applied
+// to the operands as they are read.
+typedef __attribute__((address_space(3))) void *lds_ptr;
+// LDS reads at byte address a + immediate offsets, waited for in the same
+// block: the compiler would otherwise wait for the chunk loads in flight
+// into the other buffer before every LDS read (it cannot tell the two
+// buffers apart).  Offsets: B + S2 k (four), B + S1 (k / 4) + S2 (k % 4)
+// (eight).
+template <int B, int S2>
+__device__ inline void lds_rd4(unsigned a, double (&x)[4])
+{
+    asm volatile("ds_read_b64 %0, %4 offset:%5\n\t"
+                 "ds_read_b64 %1, %4 offset:%6\n\t"
+                 "ds_read_b64 %2, %4 offset:%7\n\t"
+                 "ds_read_b64 %3, %4 offset:%8\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(x[0]), "=&v"(x[1]), "=&v"(x[2]), "=&v"(x[3])
+                 : "v"(a), "i"(B), "i"(B + S2), "i"(B + 2 * S2), "i"(B + 3 * S2)
+                 : "memory");
+}
+template <int B, int S1, int S2>
+__device__ inline void lds_rd8(unsigned a, double (&x)[8])
+{
+    asm volatile("ds_read_b64 %0, %8 offset:%9\n\t"
+                 "ds_read_b64 %1, %8 offset:%10\n\t"
+                 "ds_read_b64 %2, %8 offset:%11\n\t"
+                 "ds_read_b64 %3, %8 offset:%12\n\t"
+                 "ds_read_b64 %4, %8 offset:%13\n\t"
+                 "ds_read_b64 %5, %8 offset:%14\n\t"
+                 "ds_read_b64 %6, %8 offset:%15\n\t"
+                 "ds_read_b64 %7, %8 offset:%16\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(x[0]), "=&v"(x[1]), "=&v"(x[2]), "=&v"(x[3]), "=&v"(x[4]), "=&v"(x[5]), "=&v"(x[6]),
+                   "=&v"(x[7])
+                 : "v"(a), "i"(B), "i"(B + S2), "i"(B + 2 * S2), "i"(B + 3 * S2), "i"(B + S1), "i"(B + S1 + S2),
+                   "i"(B + S1 + 2 * S2), "i"(B + S1 + 3 * S2)
+                 : "memory");
+}
+template <bool ABS, int RT>
+__global__ __launch_bounds__(512, 1) void k_symm_part2(const double *__restrict__ A, int N, int MT,
+                                                      const double *__restrict__ X, int ldx, int nr,
+                                                      double *__restrict__ P)
+{
+    constexpr int CW = 16;                       // columns per chunk
+    constexpr int TB = RT * 128;                 // tile order
+    constexpr int CS = TB + 16;                  // LDS column stride (doubles): 128 bytes of padding
+    constexpr int RWV = RT * 16;                 // rows per wave
+    // two buffers as two objects, so that reads of one need not wait for
+    // the direct loads into the other
+    __shared__ __align__(16) double S0[CW * CS];
+    __shared__ __align__(16) double S1[CW * CS];
+    __shared__ double red[8][CW * 16 + CW];
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, w = tid >> 6, fr = lane & 15, fk = lane >> 4;
+    int t = blockIdx.x, n = 0;
+    while (t >= MT - n) { t -= MT - n; n++; }
+    const int m = n + t;
+    const bool dg = m == n;
+    const int rows_m = tile_rows(N, TB, MT, m), cols_n = tile_rows(N, TB, MT, n);
+    const bool edge = dg || rows_m < TB || cols_n < TB;      // workgroup-uniform
+    const int64_t ti = tile_index(MT, m, n);
+    const double *At = A + ti * (int64_t)TB * TB;
+    double *P1 = P + ti * (int64_t)SY_PART * TB;
+    double *P2 = P1 + 16 * TB;
+    double *A1 = P2 + 16 * TB;
+    double *A2 = A1 + TB;
+    const int jc = min(fr, nr - 1);              // clamped right-hand side of this lane
+    const double jm = fr < nr ? 1.0 : 0.0;
+    const double *Xn = X + (int64_t)jc * ldx + (int64_t)n * TB;
+    const double *Xm = X + (int64_t)jc * ldx + (int64_t)m * TB;
+    const int rw0 = w * RWV;
+
+    // X values are read at clamped (valid, finite) positions and scaled by
+    // 0 or 1, so that no load depends on a condition.  b-operands of P2:
+    // X_m(rw0 + 4 s + fk, fr)
+    double xm[4 * RT];
+#pragma unroll
+    for (int s = 0; s < 4 * RT; s++) {
+        const int r = rw0 + 4 * s + fk;
+        xm[s] = Xm[(unsigned)min(r, rows_m - 1)] * (r < rows_m ? jm : 0.0);
+    }
+    dx4 p1[RT];
+    double a1[RT];
+#pragma unroll
+    for (int i = 0; i < RT; i++) {
+        p1[i] = dx4{0.0, 0.0, 0.0, 0.0};
+        a1[i] = 0.0;
+    }
+    // chunk ch into buffer Sd: 16 RT pieces of 128 doubles (column ch CW +
+    // piece / RT, rows 128 (piece % RT) ..), two RT per wave
+    auto dload = [&](double *Sd, int ch) {
+#pragma unroll
+        for (int q = 0; q < 2 * RT; q++) {
+            const int pc = w + 8 * q, col = pc / RT, part = pc - col * RT;
+            __builtin_amdgcn_global_load_lds(At + (int64_t)(ch * CW + col) * TB + part * 128 + 2 * lane,
+                                             (lds_ptr)&Sd[col * CS + part * 128], 16, 0, 0);
+        }
+    };
+    // the products of chunk ch from Sb; masked operands (diagonal tile:
+    // P1 the lower triangle, P2 the strict lower; rows and columns past the
+    // matrix: zero) when E
+    auto compute = [&](auto edge_tag, const double *Sb, int c0, const double (&xn)[4]) {
+        constexpr bool E = decltype(edge_tag)::value;
+        // a-operands of P1, A(rw0 + 16 i + fr, c0 + 4 u + fk), as ap[4 i + u]
+        double ap[4 * RT];
+        {
+            const unsigned b1 = (unsigned)(size_t)(lds_ptr)&Sb[fk * CS + rw0 + fr];
+            if constexpr (RT == 1) {
+                double x[4];
+                lds_rd4<0, 32 * CS>(b1, x);
+#pragma unroll
+                for (int k = 0; k < 4; k++) ap[k] = x[k];
+            } else {
+#pragma unroll
+                for (int i0 = 0; i0 + 1 < RT; i0 += 2) {
+                    double x[8];
+                    if (i0 == 0) lds_rd8<0, 128, 32 * CS>(b1, x);
+                    else lds_rd8<256, 128, 32 * CS>(b1, x);
+#pragma unroll
+                    for (int k = 0; k < 8; k++) ap[4 * i0 + k] = x[k];
+                }
+                if constexpr (RT == 3) {
+                    double x[4];
+                    lds_rd4<256, 32 * CS>(b1, x);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) ap[8 + k] = x[k];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RT; i++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = rw0 + 16 * i + fr, c = 4 * u + fk;
+                double a = ap[4 * i + u];
+                if (E) a = (r < rows_m && c0 + c < cols_n && (!dg || r >= c0 + c)) ? a : 0.0;
+                if (ABS) a1[i] += fabs(a);
+                p1[i] = MF<double>::mfma(a, xn[u], p1[i]);
+            }
+        // a-operands of P2, A(rw0 + 4 s + fk, c0 + fr), as at[s]
+        double at[4 * RT];
+        {
+            const unsigned b2 = (unsigned)(size_t)(lds_ptr)&Sb[fr * CS + rw0 + fk];
+            if constexpr (RT == 1) {
+                double x[4];
+                lds_rd4<0, 32>(b2, x);
+#pragma unroll
+                for (int k = 0; k < 4; k++) at[k] = x[k];
+            } else {
+#pragma unroll
+                for (int s0 = 0; s0 + 7 < 4 * RT; s0 += 8) {
+                    double x[8];
+                    if (s0 == 0) lds_rd8<0, 128, 32>(b2, x);
+                    else lds_rd8<256, 128, 32>(b2, x);
+#pragma unroll
+                    for (int k = 0; k < 8; k++) at[s0 + k] = x[k];
+                }
+                if constexpr (RT == 3) {
+                    double x[4];
+                    lds_rd4<256, 32>(b2, x);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) at[8 + k] = x[k];
+                }
+            }
+        }
+        dx4 p2 = dx4{0.0, 0.0, 0.0, 0.0};
+        double a2 = 0.0;
+#pragma unroll
+        for (int s = 0; s < 4 * RT; s++) {
+            const int r = rw0 + 4 * s + fk;
+            double a = at[s];
+            if (E) a = (r < rows_m && c0 + fr < cols_n && (!dg || r > c0 + fr)) ? a : 0.0;
+            if (ABS) a2 += fabs(a);
+            p2 = MF<double>::mfma(a, xm[s], p2);
+        }
+        // element q of p2 in lane (fr, fk): P2(c0 + 4 q + fk, fr)
+        double *Rw = red[w];
+#pragma unroll
+        for (int q = 0; q < 4; q++) Rw[(4 * q + fk) * 16 + fr] = p2[q];
+        if (ABS) {
+            a2 += __shfl_xor(a2, 16);
+            a2 += __shfl_xor(a2, 32);
+            if (fk == 0) Rw[CW * 16 + fr] = a2;
+        }
+    };
+    const int nch = (cols_n + CW - 1) / CW;
+    // b-operands of P1 for chunk ch, X_n(16 ch + 4 u + fk, fr), read one
+    // chunk ahead, before that chunk's own loads
+    double xr[4];
+    auto read_xn = [&](int ch) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) xr[u] = Xn[(unsigned)min(ch * CW + 4 * u + fk, cols_n - 1)];
+    };
+    auto step = [&](int ch, const double *Sb, double *Sn) {
+        const int c0 = ch * CW;
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");      // this wave's loads of chunk ch and its xr
+        __syncthreads();                                      // everyone's; the other buffer free
+        double xn[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) xn[u] = xr[u] * (c0 + 4 * u + fk < cols_n ? jm : 0.0);
+        // xn in registers before the loads below are issued, so that no
+        // wait for it waits for them
+        asm volatile("" : "+v"(xn[0]), "+v"(xn[1]), "+v"(xn[2]), "+v"(xn[3])::"memory");
+        // the next chunk; after the last one, this chunk again (from L2)
+        read_xn(min(ch + 1, nch - 1));
+        dload(Sn, min(ch + 1, nch - 1));
+        if (edge) compute(std::true_type{}, Sb, c0, xn);
+        else compute(std::false_type{}, Sb, c0, xn);
+        // a barrier without the memory fence of __syncthreads, which would
+        // wait for the loads just issued: the LDS writes of red are waited
+        // for explicitly, and red is read back by waited asm reads
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (tid < CW * 16 + (ABS ? CW : 0)) {
+            double v[8];
+            lds_rd8<0, 4 * (CW * 16 + CW) * 8, (CW * 16 + CW) * 8>((unsigned)(size_t)(lds_ptr)&red[0][tid], v);
+            const double sum = ((((((v[0] + v[1]) + v[2]) + v[3]) + v[4]) + v[5]) + v[6]) + v[7];
+            if (tid < CW * 16) P2[(int64_t)(c0 + (tid >> 4)) * 16 + (tid & 15)] = sum;
+            else A2[c0 + tid - CW * 16] = sum;
+        }
+    };
+    read_xn(0);
+    dload(S0, 0);
+    for (int ch = 0; ch < nch; ch += 2) {
+        step(ch, S0, S1);
+        if (ch + 1 < nch) step(ch + 1, S1, S0);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");          // the last, unused loads
+    // element q of p1[i] in lane (fr, fk): P1(rw0 + 16 i + 4 q + fk, fr)
+#pragma unroll
+    for (int i = 0; i < RT; i++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) P1[(int64_t)(rw0 + 16 * i + MF<double>::row(q, fk)) * 16 + fr] = p1[i][q];
+        if (ABS) {
+            double v = a1[i];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (fk == 0) A1[rw0 + 16 * i + fr] = v;
+        }
+    }
+}
+
 // R (ldr) = B (ldb) - (P1 + P2 sums) for nr right-hand sides; thread = (row,
 // right-hand side), partials summed over n = 0..m, then m' = m..MT-1.
 __global__ __launch_bounds__(256) void k_symm_sum(const double *__restrict__ P, int N, int NB, int MT,
@@ -494,6 +748,8 @@ __global__ void k_flag_clear(int *p) { *p = 0; }
 
 namespace mi {
 
+int g_symm_kernel = 1;   // 1: k_symm_part2 when NB % 128 == 0; 0: k_symm_part
+
 // R = B - A X for NRHS columns (chunks of 16), enqueued on s.
 int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS, const double *dX, int ldx,
                 const double *dB, int ldb, double *dR, int ldr, double *rowabs)
@@ -509,7 +765,27 @@ int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS
             const int nr = NRHS - j0 < 16 ? NRHS - j0 : 16;
             double *ra = j0 == 0 ? rowabs : nullptr;
             const double *Xj = dX + (int64_t)j0 * ldx;
-            if (ra)
+            if (NB % 128 == 0 && g_symm_kernel) {
+                // whole-chunk LDS loads (k_symm_part2), NB = 128 .. 512
+                const double fl = 2.0 * N * (double)N * nr, by = 8.0 * (double)ntile * NB * NB;
+                const unsigned gr = (unsigned)ntile;
+#define MI_SYMM2(RT)                                                                                             \
+    do {                                                                                                         \
+        if (ra)                                                                                                  \
+            MI_LAUNCH(c, s, "dsymm_part", fl, by,                                                                \
+                      k_symm_part2<true, RT><<<gr, 512, 0, s>>>(dA, N, MT, Xj, ldx, nr, P));                    \
+        else                                                                                                     \
+            MI_LAUNCH(c, s, "dsymm_part", fl, by,                                                                \
+                      k_symm_part2<false, RT><<<gr, 512, 0, s>>>(dA, N, MT, Xj, ldx, nr, P));                   \
+    } while (0)
+                switch (NB / 128) {
+                case 1: MI_SYMM2(1); break;
+                case 2: MI_SYMM2(2); break;
+                case 3: MI_SYMM2(3); break;
+                default: MI_SYMM2(4); break;
+                }
+#undef MI_SYMM2
+            } else if (ra)
                 MI_LAUNCH(c, s, "dsymm_part", 2.0 * N * (double)N * nr, 8.0 * (double)ntile * NB * NB,
                           k_symm_part<true><<<(unsigned)ntile, 256, 0, s>>>(dA, N, NB, MT, Xj, ldx, nr, P));
             else
@@ -535,6 +811,13 @@ int dsymm_resid(Ctx *c, hipStream_t s, int N, int NB, const double *dA, int NRHS
 }  // namespace mi
 
 extern "C" {
+
+int mi_set_symm_kernel(int which)
+{
+    if (which != 0 && which != 1) { set_error("residual kernel must be 0 (register operands) or 1 (LDS chunks)"); return -2; }
+    g_symm_kernel = which;
+    return 0;
+}
 
 int mi_dsymm_resid(mi_ctx_t *c, int N, int NB, const double *dA, int NRHS, const double *dX, int ldx,
                    const double *dB, int ldb, double *dR, int ldr)

@@ -212,6 +212,12 @@ int mi_set_spotrf_update(int which);
  * splits the FP32 panel tiles it reads.  Both give the same factor bit for
  * bit.  Process-wide. */
 int mi_set_bf16_panel_split(int which);
+/* Kernel of the FP64 residual R = B - A X (mi_dsymm_resid and the
+ * refinement of mi_dsposv_tiles) for 64 <= NB <= 512: 1 = whole 16-column
+ * chunks of each tile loaded straight into LDS, for NB a multiple of 128
+ * (default), 0 = operands loaded into registers.  Both read every stored
+ * element once; they sum in different orders.  Process-wide. */
+int mi_set_symm_kernel(int which);
 
 /* ------------------------------------------------------------ task insertion */
 /* Task-insertion engine behind the DTD front-end of dplasma.h (the tile

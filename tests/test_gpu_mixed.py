@@ -37,10 +37,16 @@ def nan_unreferenced(po, T, N, NB):
     return T
 
 
+@pytest.mark.parametrize("kernel", [1, 0], ids=["lds_chunks", "registers"])
 @pytest.mark.parametrize("N,NB,nrhs", [(378, 93, 16), (1000, 256, 5), (700, 64, 1), (1100, 512, 17),
                                        (1024, 512, 16), (300, 32, 16), (1500, 640, 3),
-                                       (777, 96, 16), (1000, 250, 7)])
-def test_dsymm_resid(dev, po, N, NB, nrhs):
+                                       (777, 96, 16), (1000, 250, 7), (1000, 128, 16), (1300, 384, 16),
+                                       (2100, 512, 3), (130, 128, 2)])
+def test_dsymm_resid(dev, po, N, NB, nrhs, kernel):
+    """R = B - A X on the packed lower tiles (NaN in every unreferenced
+    element); NB = 128, 256, 384, 512 run the LDS-chunk kernel when it is
+    selected, the others the register kernel both times."""
+    dev.check(dev.M.mi_set_symm_kernel(kernel), "symm kernel")
     rng = np.random.default_rng(N)
     A = rng.standard_normal((N, N))
     A = A + A.T
@@ -50,7 +56,10 @@ def test_dsymm_resid(dev, po, N, NB, nrhs):
     dT, dX, dB = dev.upload(T), dev.upload(np.asfortranarray(X).reshape(-1, order="F")), \
         dev.upload(np.asfortranarray(B).reshape(-1, order="F"))
     dR = dev.alloc(N * nrhs * 8)
-    dev.check(dev.M.mi_dsymm_resid(dev.h, N, NB, dT, nrhs, dX, N, dB, N, dR, N), "resid")
+    try:
+        dev.check(dev.M.mi_dsymm_resid(dev.h, N, NB, dT, nrhs, dX, N, dB, N, dR, N), "resid")
+    finally:
+        dev.M.mi_set_symm_kernel(1)
     R = dev.download(dR, (N * nrhs,)).reshape((N, nrhs), order="F")
     for p in (dT, dX, dB, dR):
         dev.free(p)
