@@ -706,27 +706,49 @@ __global__ __launch_bounds__(256) void k_symm_sum(const double *__restrict__ P, 
 }
 
 // out[j] = max_i |V(i, j)| for the nc columns of V (M x nc, ld); NaN wins.
-// One workgroup per column.
-__global__ __launch_bounds__(256) void k_colamax(const double *__restrict__ V, int ld, int M,
+// Grid (ceil(M / 4096), nc): each workgroup reduces 4096 rows of one column
+// and folds its maximum into out[j] (cleared by k_amax_clear) with a 64-bit
+// atomic maximum on the bit pattern: for non-negative values the integer
+// order of IEEE doubles is the numeric order, and a NaN with its sign
+// cleared lies above +Inf.  The maximum does not depend on the order.
+constexpr int AMAX_ROWS = 4096;
+__global__ __launch_bounds__(256) void k_amax_clear(double *__restrict__ out, int nc)
+{
+    for (int j = threadIdx.x; j < nc; j += 256) out[j] = 0.0;
+}
+__global__ __launch_bounds__(256) void k_colamax(const double *__restrict__ V, int ld, int M, int nc,
                                                 double *__restrict__ out)
 {
-    __shared__ double sm[256];
-    const double *v = V + (int64_t)blockIdx.x * ld;
-    double mx = 0.0;
-    for (int i = threadIdx.x; i < M; i += 256) {
-        const double a = fabs(v[i]);
-        mx = (a != a || mx != mx) ? NAN : fmax(mx, a);
-    }
-    sm[threadIdx.x] = mx;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            const double a = sm[threadIdx.x], b = sm[threadIdx.x + w];
-            sm[threadIdx.x] = (a != a || b != b) ? NAN : fmax(a, b);
+    __shared__ unsigned long long sm[4];
+    const int i0 = blockIdx.x * AMAX_ROWS;
+    const int i1 = min(M, i0 + AMAX_ROWS);
+    for (int j = blockIdx.y; j < nc; j += gridDim.y) {
+        const double *v = V + (int64_t)j * ld;
+        unsigned long long mx = 0;
+        for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(v[i]) & 0x7FFFFFFFFFFFFFFFull;
+            mx = b > mx ? b : mx;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long u = __shfl_xor(mx, o);
+            mx = u > mx ? u : mx;
+        }
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < 4; w++) mx = sm[w] > mx ? sm[w] : mx;
+            atomicMax((unsigned long long *)&out[j], mx);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
+// out[0 .. nc) = column maxima of |V| (M x nc, ld), enqueued on s
+inline void colamax(hipStream_t s, const double *V, int ld, int M, int nc, double *out)
+{
+    k_amax_clear<<<1, 256, 0, s>>>(out, nc);
+    k_colamax<<<dim3((unsigned)((M + AMAX_ROWS - 1) / AMAX_ROWS), (unsigned)std::min(nc, 65535)), 256, 0, s>>>(
+        V, ld, M, nc, out);
 }
 
 __global__ void k_flag_clear(int *p) { *p = 0; }
@@ -926,9 +948,9 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
             // the first residual also yields the row sums of |A|, hence ||A||inf
             if ((rc = dsymm_resid(c, s, N, NB, dA, NRHS, dX, ldx, dB, ldb, R, N, it == 0 ? rs : nullptr)))
                 goto done;
-            k_colamax<<<NRHS, 256, 0, s>>>(R, N, N, nrm);
-            k_colamax<<<NRHS, 256, 0, s>>>(dX, ldx, N, nrm + NRHS);
-            if (it == 0) k_colamax<<<1, 256, 0, s>>>(rs, N, N, nrm + 2 * NRHS);
+            colamax(s, R, N, N, NRHS, nrm);
+            colamax(s, dX, ldx, N, NRHS, nrm + NRHS);
+            if (it == 0) colamax(s, rs, N, N, 1, nrm + 2 * NRHS);
             MX_CHECK(hipMemcpyAsync(h.data(), nrm, (2 * NRHS + 1) * sizeof(double), hipMemcpyDeviceToHost, s),
                      done);
             MX_CHECK(hipMemcpyAsync(&h_ovf, c->d_info, sizeof(int), hipMemcpyDeviceToHost, s), done);
