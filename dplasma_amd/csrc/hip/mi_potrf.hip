@@ -862,6 +862,30 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 constexpr int XK = 32, XS = XK + 8;
 
+// Epilogue of the BF16-split updates: the TI x 4 accumulator tiles of a wave
+// (rows r0 + 16 i + 4 fk + q, columns c0 + 16 j + fr) stored as C (pre:
+// C was the MFMA's start; 16-byte stores of four rows) or added to C with
+// the tile and lower-triangle masks (diag: a diagonal block).
+template <int TI>
+__device__ inline void bf16x3_store(float *Cp, int NB, const fx4 (&acc)[TI][4], int r0, int c0, int fr, int fk,
+                                    bool pre, int rows_m, int cols_n, bool diag)
+{
+#pragma unroll
+    for (int i = 0; i < TI; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = r0 + i * 16 + 4 * fk, col = c0 + j * 16 + fr;
+            float *c = &Cp[(int64_t)col * NB + row];
+            if (pre) {
+                *(fx4 *)c = acc[i][j];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (row + q < rows_m && col < cols_n && (!diag || row + q >= col)) c[q] += acc[i][j][q];
+            }
+        }
+}
+
 __global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt, int N, int NB, int MT, int k,
                                                          int n_begin, int m_off, int nsub)
 {
@@ -887,6 +911,8 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt
     float *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
     const int fr = lane & 15, fk = lane >> 4;
 
+    // element q of acc[i][j] in lane (fr, fk): C(row 4 fk + q of row tile i,
+    // column fr of column tile j), four consecutive rows of one column
     fx4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -897,12 +923,9 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int row = r_base + wm * 64 + i * 16 + fr;
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    acc[i][j][q] = Cp[(int64_t)(c_base + wn * 64 + j * 16 + 4 * fk + q) * NB + row];
-            }
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = *(const fx4 *)&Cp[(int64_t)(c_base + wn * 64 + j * 16 + fr) * NB + r_base + wm * 64 +
+                                              i * 16 + 4 * fk];
     }
 
     // staging: thread (rq, kq) = (tid % 32, tid / 32) holds rows rq + 32 i
@@ -971,27 +994,13 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3(float *__restrict__ Tt
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                // fed (B, A): C^T, so that lanes walk rows in the epilogue
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
             }
         __syncthreads();
     }
-
-    const bool diag = (m == n) && (bm == bn);
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int rr = r_base + wm * 64 + i * 16 + fr;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int col = c_base + wn * 64 + j * 16 + 4 * fk + q;
-                if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
-                else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
-            }
-    }
+    bf16x3_store(Cp, NB, acc, r_base + wm * 64, c_base + wn * 64, fr, fk, pre, rows_m, cols_n, (m == n) && (bm == bn));
 }
 
 // ---------------------------------------------------------------------------
@@ -1077,7 +1086,7 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3p(float *__restrict__ T
     float *Cp = Tt + tile_index(MT, m, n) * (int64_t)NB * NB;
     const int fr = lane & 15, fk = lane >> 4;
 
-    fx4 acc[4][4];
+    fx4 acc[4][4];                               // as in k_update_bf16x3
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -1087,12 +1096,9 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3p(float *__restrict__ T
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int row = r_base + wm * 64 + i * 16 + fr;
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    acc[i][j][q] = Cp[(int64_t)(c_base + wn * 64 + j * 16 + 4 * fk + q) * NB + row];
-            }
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = *(const fx4 *)&Cp[(int64_t)(c_base + wn * 64 + j * 16 + fr) * NB + r_base + wm * 64 +
+                                              i * 16 + 4 * fk];
     }
 
     typedef unsigned int u4 __attribute__((ext_vector_type(4)));
@@ -1142,9 +1148,9 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3p(float *__restrict__ T
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
             }
     };
 
@@ -1168,19 +1174,7 @@ __global__ __launch_bounds__(256, 2) void k_update_bf16x3p(float *__restrict__ T
         }
     }
 
-    const bool diag = (m == n) && (bm == bn);
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int rr = r_base + wm * 64 + i * 16 + fr;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int col = c_base + wn * 64 + j * 16 + 4 * fk + q;
-                if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
-                else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
-            }
-    }
+    bf16x3_store(Cp, NB, acc, r_base + wm * 64, c_base + wn * 64, fr, fk, pre, rows_m, cols_n, (m == n) && (bm == bn));
 }
 
 // k_update_bf16x3p with 256 x 256 blocks (NB a multiple of 256): half the
@@ -1235,8 +1229,14 @@ __global__ __launch_bounds__(512, 1) void k_update_bf16x3w(float *__restrict__ T
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                acc[i][j][q] = pre ? Cp[(int64_t)(c_base + wn * 64 + j * 16 + 4 * fk + q) * NB + pr + i * 16 + fr]
-                                   : 0.f;
+                acc[i][j][q] = 0.f;
+    if (pre) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = *(const fx4 *)&Cp[(int64_t)(c_base + wn * 64 + j * 16 + fr) * NB + pr + i * 16 + 4 * fk];
+    }
     typedef unsigned int u4 __attribute__((ext_vector_type(4)));
     auto neg = [](bf8 v) -> bf8 { return __builtin_bit_cast(bf8, __builtin_bit_cast(u4, v) ^ 0x80008000u); };
     // wave w fetches rows 32 w .. 32 w + 31 of the four images
@@ -1272,28 +1272,16 @@ __global__ __launch_bounds__(512, 1) void k_update_bf16x3w(float *__restrict__ T
             const bf8 ah = *(const bf8 *)&S[b][0][o], al = *(const bf8 *)&S[b][1][o];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
             }
         }
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");      // this wave's loads of step st + 1
         __syncthreads();                                      // everyone's, and buffer b read
     }
 
-    const bool diag = m == n;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const int rr = pr + i * 16 + fr;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int col = c_base + wn * 64 + j * 16 + 4 * fk + q;
-                if (pre) Cp[(int64_t)col * NB + rr] = acc[i][j][q];
-                else if (rr < rows_m && col < cols_n && (!diag || rr >= col)) Cp[(int64_t)col * NB + rr] += acc[i][j][q];
-            }
-    }
+    bf16x3_store(Cp, NB, acc, pr, c_base + wn * 64, fr, fk, pre, rows_m, cols_n, m == n);
 }
 
 // ---------------------------------------------------------------------------
