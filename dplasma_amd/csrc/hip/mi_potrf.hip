@@ -39,6 +39,16 @@ namespace {
 constexpr int PJB = 32;            // panel width inside the diagonal factorization
 constexpr int POTRF_THREADS = 512;
 
+// v = *p, kept where c holds and +0 elsewhere; p must be a valid address
+// whatever c is.  The load is unconditional, so that the compiler issues a
+// group of them together instead of branching around each and waiting for
+// it on its own.
+template <class T> __device__ __forceinline__ T load_if(const T *p, bool c)
+{
+    const T v = *p;
+    return c ? v : T(0);
+}
+
 // ---------------------------------------------------------------------------
 // One wave: acc[ct] += X[r0+i][0:K] * Y[c0+16ct+j][0:K]^T for the 16x16 output
 // tiles ct = 0, 1 (i, j < 16) on v_mfma_f64_16x16x4_f64, operands read straight
@@ -55,6 +65,8 @@ __device__ inline void mfma_panel_2(const T *__restrict__ X, int ldx, int r0, in
     const int lane = threadIdx.x & 63, fr = lane & 15, fk = lane >> 4;
     const int xr = r0 + fr, y0 = c0 + fr, y1 = c0 + 16 + fr;
     const bool xv = xr < xr_lim, yv0 = y0 < yr_lim, yv1 = y1 < yr_lim;
+    // rows past the limits read the last valid row (xr_lim, yr_lim >= 1)
+    const int xrc = min(xr, xr_lim - 1), y0c = min(y0, yr_lim - 1), y1c = min(y1, yr_lim - 1);
     V4 e0 = {0, 0, 0, 0}, e1 = e0, o0 = e0, o1 = e0;
     // 32 columns of K per pass; the operands of pass p+1 are in flight while
     // the 16 MFMAs of pass p run
@@ -63,9 +75,17 @@ __device__ inline void mfma_panel_2(const T *__restrict__ X, int ldx, int r0, in
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int64_t kk = k + 4 * s + fk;
-            pa[s] = xv ? X[kk * ldx + xr] : T(0);
-            pb0[s] = yv0 ? Y[kk * ldy + y0] : T(0);
-            pb1[s] = yv1 ? Y[kk * ldy + y1] : T(0);
+            if constexpr (sizeof(T) == 4) {
+                pa[s] = load_if(&X[kk * ldx + xrc], xv);
+                pb0[s] = load_if(&Y[kk * ldy + y0c], yv0);
+                pb1[s] = load_if(&Y[kk * ldy + y1c], yv1);
+            } else {
+                // FP64: all of them in flight would spill (45 registers in
+                // k_trsm_tile), which costs more than the waits
+                pa[s] = xv ? X[kk * ldx + xr] : T(0);
+                pb0[s] = yv0 ? Y[kk * ldy + y0] : T(0);
+                pb1[s] = yv1 ? Y[kk * ldy + y1] : T(0);
+            }
         }
     };
     if (K > 0) load(0, a, b0, b1);
@@ -108,7 +128,7 @@ __device__ inline void syrk_unit32(T *__restrict__ A, int ld, int n, const T *__
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int r = r0 + 16 * i + fr, c = c0 + 16 * j + MF<T>::row(q, fk);
-                acc[i][j][q] = (r < n && c < n && r >= c) ? A[(int64_t)c * ld + r] : T(0);
+                acc[i][j][q] = load_if(&A[(int64_t)min(c, n - 1) * ld + min(r, n - 1)], r < n && c < n && r >= c);
             }
     const int ra0 = min(r0 + fr, RC - 1), ra1 = min(r0 + 16 + fr, RC - 1);
     const int cb0 = min(c0 + fr, RC - 1), cb1 = min(c0 + 16 + fr, RC - 1);
@@ -184,7 +204,8 @@ __device__ inline void potrf_tile_dev(T *__restrict__ A, int n, int ld, int RC, 
 #pragma unroll
         for (int c = 0; c < 32; c++) {
             const int r = j0 + lane, cc = j0 + c;
-            const T a = from_lds ? Dsm[c * 33 + lr] : ((r < n && cc < n) ? A[(int64_t)cc * ld + r] : T(0));
+            const T a = from_lds ? Dsm[c * 33 + lr]
+                                 : load_if(&A[(int64_t)min(cc, n - 1) * ld + min(r, n - 1)], r < n && cc < n);
             v[c] = lrow ? ((r < n && cc < n) ? a : T(0)) : (c == j ? T(1) : T(0));
         }
     };
@@ -239,7 +260,7 @@ __device__ inline void potrf_tile_dev(T *__restrict__ A, int n, int ld, int RC, 
 #pragma unroll
             for (int s4 = 0; s4 < 8; s4++) {
                 const int kk = 4 * s4 + fk;
-                const T a = (r < n && kk < jw) ? A[(int64_t)(j0 + kk) * ld + rc] : T(0);
+                const T a = load_if(&A[(int64_t)(j0 + min(kk, jw - 1)) * ld + rc], r < n && kk < jw);
                 e0 = MF<T>::mfma(Ws[kk * 32 + fr], a, e0);          // W(c, kk), c = fr
                 e1 = MF<T>::mfma(Ws[kk * 32 + 16 + fr], a, e1);     // c = 16 + fr
             }
@@ -316,7 +337,7 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile(T *__restrict__ Tt, int N,
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int rr = wave * 16 + MF<T>::row(q, fk), c = ct * 16 + fr;
-                cb[ct][q] = (rr < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + rr] : T(0);
+                cb[ct][q] = load_if(&C[(int64_t)(j0 + min(c, jw - 1)) * NB + min(rr, rv - 1)], rr < rv && c < jw);
             }
 #pragma unroll
         for (int u = 0; u < 4; u++) wv[u] = Winv[(j0 >> 5) * 1024 + tid + 256 * u];
@@ -391,7 +412,7 @@ __global__ __launch_bounds__(256, 2) void k_trsm_tile_ks(T *__restrict__ Tt, int
 #pragma unroll
         for (int u = 0; u < 2; u++) {
             const int idx = tid + 256 * u, r = idx % TSK_ROWS, c = idx / TSK_ROWS;
-            cb[u] = (r < rv && c < jw) ? C[(int64_t)(j0 + c) * NB + r] : T(0);
+            cb[u] = load_if(&C[(int64_t)(j0 + min(c, jw - 1)) * NB + min(r, rv - 1)], r < rv && c < jw);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) wv[u] = Winv[(j0 >> 5) * 1024 + tid + 256 * u];
