@@ -68,31 +68,20 @@ __device__ inline void mfma_panel_2(const T *__restrict__ X, int ldx, int r0, in
     // rows past the limits read the last valid row (xr_lim, yr_lim >= 1)
     const int xrc = min(xr, xr_lim - 1), y0c = min(y0, yr_lim - 1), y1c = min(y1, yr_lim - 1);
     V4 e0 = {0, 0, 0, 0}, e1 = e0, o0 = e0, o1 = e0;
-    // 32 columns of K per pass; the operands of pass p+1 are in flight while
-    // the 16 MFMAs of pass p run
+    // 32 columns of K per pass, all 24 loads of a pass issued together; in
+    // FP32 the operands of pass p+1 are in flight while the 16 MFMAs of pass
+    // p run
     T a[8], b0[8], b1[8];
     auto load = [&](int k, T *pa, T *pb0, T *pb1) {
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int64_t kk = k + 4 * s + fk;
-            if constexpr (sizeof(T) == 4) {
-                pa[s] = load_if(&X[kk * ldx + xrc], xv);
-                pb0[s] = load_if(&Y[kk * ldy + y0c], yv0);
-                pb1[s] = load_if(&Y[kk * ldy + y1c], yv1);
-            } else {
-                // FP64: all of them in flight would spill (45 registers in
-                // k_trsm_tile), which costs more than the waits
-                pa[s] = xv ? X[kk * ldx + xr] : T(0);
-                pb0[s] = yv0 ? Y[kk * ldy + y0] : T(0);
-                pb1[s] = yv1 ? Y[kk * ldy + y1] : T(0);
-            }
+            pa[s] = load_if(&X[kk * ldx + xrc], xv);
+            pb0[s] = load_if(&Y[kk * ldy + y0c], yv0);
+            pb1[s] = load_if(&Y[kk * ldy + y1c], yv1);
         }
     };
-    if (K > 0) load(0, a, b0, b1);
-    for (int k = 0; k < K; k += 32) {
-        T na[8], nb0[8], nb1[8];
-        const bool more = k + 32 < K;
-        if (more) load(k + 32, na, nb0, nb1);
+    auto mul = [&]() {
 #pragma unroll
         for (int s = 0; s < 8; s += 2) {
             e0 = MF<T>::mfma(a[s], b0[s], e0);
@@ -100,9 +89,25 @@ __device__ inline void mfma_panel_2(const T *__restrict__ X, int ldx, int r0, in
             o0 = MF<T>::mfma(a[s + 1], b0[s + 1], o0);
             o1 = MF<T>::mfma(a[s + 1], b1[s + 1], o1);
         }
-        if (more) {
+    };
+    if constexpr (sizeof(T) == 8) {
+        // FP64: one chunk of 24 loads in flight (two would spill 45
+        // registers in k_trsm_tile)
+        for (int k = 0; k < K; k += 32) {
+            load(k, a, b0, b1);
+            mul();
+        }
+    } else {
+        if (K > 0) load(0, a, b0, b1);
+        for (int k = 0; k < K; k += 32) {
+            T na[8], nb0[8], nb1[8];
+            const bool more = k + 32 < K;
+            if (more) load(k + 32, na, nb0, nb1);
+            mul();
+            if (more) {
 #pragma unroll
-            for (int s = 0; s < 8; s++) { a[s] = na[s]; b0[s] = nb0[s]; b1[s] = nb1[s]; }
+                for (int s = 0; s < 8; s++) { a[s] = na[s]; b0[s] = nb0[s]; b1[s] = nb1[s]; }
+            }
         }
     }
     acc[0] += e0 + o0;
