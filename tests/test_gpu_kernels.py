@@ -409,16 +409,17 @@ def test_s_trailing_update_bf16x3(dev, po, N, NB):
 
 
 @pytest.mark.parametrize("N,NB", [(1536, 512), (1700, 320), (4096, 256), (2000, 128), (378, 93),
-                                  (9512, 512), (12000, 256)])
+                                  (9512, 512), (12000, 256), (11000, 1024)])
 def test_bf16_panel_split_same_factor(dev, po, N, NB):
     """The BF16-split FP32 factor whose main-stream updates load each panel
     column split once per step into hi/lo BF16 images, against the factor
     whose update workgroups split the FP32 panel tiles themselves: the same
     bit for bit (the split of -a is minus the split of a, and rounding to
     nearest is symmetric in sign).  NB = 93 is not a multiple of 32 and
-    takes the per-workgroup split both times.  At (9512, 512) and
-    (12000, 256) the early main-stream launches cover enough tiles for the
-    256 x 256 block kernel (ragged last tiles of 296 and 224 rows)."""
+    takes the per-workgroup split both times.  At (9512, 512), (12000, 256)
+    and (11000, 1024) the early main-stream launches cover enough tiles for
+    the 256 x 256 block kernel (ragged last tiles of 296, 224 and 760 rows;
+    four blocks a side at NB = 1024)."""
     T = po.dplgsy_symtiles(N, NB, float(N), 3872).astype(np.float32)
     out = []
     dev.check(dev.M.mi_set_spotrf_update(1), "bf16x3")
