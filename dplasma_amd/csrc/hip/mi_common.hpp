@@ -128,6 +128,8 @@ struct Ctx {
     std::vector<hipEvent_t> prof_free;
     std::vector<ProfAgg> agg;
     hipEvent_t t0 = nullptr, t1 = nullptr;  // span of the last blocking call
+    double *h_stat = nullptr;       // page-locked host copy of small device results
+    size_t h_stat_n = 0;            //   (doubles), so that their copy is asynchronous
     double last_ms = 0.0;
     int prof_calls = 0;             // blocking calls whose launches were collected
 };
@@ -150,17 +152,26 @@ void prof_end(Ctx *c, hipStream_t s, ProfRec *r);
 // trsm_enqueue applies op(L)^{-1} (trans 111 / 112) to NRHS columns.
 size_t trsm_ws_elems(int N, int NB);
 template <class T> int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int NB, const T *dT, T *W);
+// first_reset_done: the hand-off memory of the first chunk of 16 columns
+// was already reset on s by trsm_reset (after the last sweep that used it).
 template <class T>
 int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, const T *W, int NRHS, T *dB,
-                 int ldb);
+                 int ldb, bool first_reset_done = false);
+template <class T> int trsm_reset(Ctx *c, hipStream_t s, int N, int NB);
 // Before the first trsm_enqueue of a blocking call: clears the abort word of
 // the wavefront solve on s; after synchronizing: -5 if a hand-off wait gave
 // up (results invalid), else 0.
 int trsm_begin(Ctx *c, hipStream_t s);
 int trsm_status(Ctx *c);
+// the abort word itself (device memory, valid after trsm_begin), and the
+// error that trsm_status reports when it is set
+const int *trsm_abort_word(Ctx *c);
+int trsm_abort_error();
 // FP32 tile factorization (mi_spotrf_tiles) with the trailing-update engine
 // given per call: bf16_split = BF16 MFMA on the two-term split.
-int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split);
+// wait = false: enqueue only (both streams joined into s_main); *info is
+// then 0 and the factorization's info is left in d_info[0].
+int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split, bool wait = true);
 // Reads, once per device and with that device current, the device address of
 // the fixed scratch of mi_dpotrf_tiles_noalloc (mi_potrf.hip).
 void noalloc_setup(int device);

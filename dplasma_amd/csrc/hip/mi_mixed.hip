@@ -632,6 +632,18 @@ __global__ __launch_bounds__(512, 1) void k_symm_part2(const double *__restrict_
 #pragma unroll
         for (int u = 0; u < 4; u++) xr[u] = Xn[(unsigned)min(ch * CW + 4 * u + fk, cols_n - 1)];
     };
+    // P2 (or A2) sum of the previous chunk, held in a register by the
+    // threads tid < CW * 16 (+ CW) and stored at the start of the next step,
+    // before that step's loads: the wait for those loads then never waits
+    // for a store issued just before it
+    const bool red_thread = tid < CW * 16 + (ABS ? CW : 0);
+    double pend = 0.0;
+    auto store_pend = [&](int c0) {
+        if (red_thread) {
+            if (tid < CW * 16) P2[(int64_t)(c0 + (tid >> 4)) * 16 + (tid & 15)] = pend;
+            else A2[c0 + tid - CW * 16] = pend;
+        }
+    };
     auto step = [&](int ch, const double *Sb, double *Sn) {
         const int c0 = ch * CW;
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");      // this wave's loads of chunk ch and its xr
@@ -642,6 +654,7 @@ __global__ __launch_bounds__(512, 1) void k_symm_part2(const double *__restrict_
         // xn in registers before the loads below are issued, so that no
         // wait for it waits for them
         asm volatile("" : "+v"(xn[0]), "+v"(xn[1]), "+v"(xn[2]), "+v"(xn[3])::"memory");
+        if (ch > 0) store_pend(c0 - CW);
         // the next chunk; after the last one, this chunk again (from L2)
         read_xn(min(ch + 1, nch - 1));
         dload(Sn, min(ch + 1, nch - 1));
@@ -653,12 +666,10 @@ __global__ __launch_bounds__(512, 1) void k_symm_part2(const double *__restrict_
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
-        if (tid < CW * 16 + (ABS ? CW : 0)) {
+        if (red_thread) {
             double v[8];
             lds_rd8<0, 4 * (CW * 16 + CW) * 8, (CW * 16 + CW) * 8>((unsigned)(size_t)(lds_ptr)&red[0][tid], v);
-            const double sum = ((((((v[0] + v[1]) + v[2]) + v[3]) + v[4]) + v[5]) + v[6]) + v[7];
-            if (tid < CW * 16) P2[(int64_t)(c0 + (tid >> 4)) * 16 + (tid & 15)] = sum;
-            else A2[c0 + tid - CW * 16] = sum;
+            pend = ((((((v[0] + v[1]) + v[2]) + v[3]) + v[4]) + v[5]) + v[6]) + v[7];
         }
     };
     read_xn(0);
@@ -667,6 +678,7 @@ __global__ __launch_bounds__(512, 1) void k_symm_part2(const double *__restrict_
         step(ch, S0, S1);
         if (ch + 1 < nch) step(ch + 1, S1, S0);
     }
+    if (nch > 0) store_pend((nch - 1) * CW);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");          // the last, unused loads
     // element q of p1[i] in lane (fr, fk): P1(rw0 + 16 i + 4 q + fk, fr)
 #pragma unroll
@@ -752,6 +764,18 @@ inline void colamax(hipStream_t s, const double *V, int ld, int M, int nc, doubl
 }
 
 __global__ void k_flag_clear(int *p) { *p = 0; }
+
+// Status words of one refinement step, as doubles next to the norms (one
+// copy to the host): the rounding of A overflowed, the rounding of B or of a
+// residual overflowed, the FP32 factorization's info, a hand-off wait of the
+// solve gave up (abort may be null: no wavefront solve).
+__global__ void k_mx_status(double *st, const int *finfo, const int *a_ovf, const int *r_ovf, const int *abort)
+{
+    st[0] = (double)*a_ovf;
+    st[1] = (double)*r_ovf;
+    st[2] = (double)*finfo;
+    st[3] = abort ? (double)*abort : 0.0;
+}
 
 // HIP check that records the error and jumps to the cleanup label
 #define MX_CHECK(call, label)                                                     \
@@ -887,15 +911,30 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
     // norms + row sums, FP32 solve workspace
     float *L32 = (float *)ws_get(c, WS_MX_L32, (size_t)ntile_el * sizeof(float));
     float *R32 = L32 ? (float *)ws_get(c, WS_MX_R32, (size_t)N * NRHS * sizeof(float)) : nullptr;
-    double *R = R32 ? (double *)ws_get(c, WS_MX_R64, ((size_t)N * NRHS + 2 * NRHS + 1 + (size_t)N) * sizeof(double))
+    const int NST = 2 * NRHS + 5;                 // norms and status words
+    double *R = R32 ? (double *)ws_get(c, WS_MX_R64, ((size_t)N * NRHS + NST + (size_t)N) * sizeof(double))
                     : nullptr;
     float *W32 = R ? (float *)ws_get(c, WS_MX_W32, trsm_ws_elems(N, NB) * sizeof(float)) : nullptr;
     if (!W32) return -3;
-    double *nrm = R + (size_t)N * NRHS;           // [0, NRHS): ||r_j||, [NRHS, 2 NRHS): ||x_j||, ||A||
-    double *rs = nrm + 2 * NRHS + 1;              // row sums of |A|
-    std::vector<double> h(2 * NRHS + 1);
+    // [0, NRHS): ||r_j||, [NRHS, 2 NRHS): ||x_j||, ||A||, then the status
+    // words of k_mx_status
+    double *nrm = R + (size_t)N * NRHS;
+    double *rs = nrm + NST;                       // row sums of |A|
+    // page-locked host copy of nrm, so that one asynchronous copy and one
+    // synchronization per refinement step bring everything the host decides on
+    if ((size_t)NST > c->h_stat_n) {
+        if (c->h_stat) MI_CHECK(hipHostFree(c->h_stat));
+        c->h_stat = nullptr;
+        c->h_stat_n = 0;
+        MI_CHECK(hipHostMalloc((void **)&c->h_stat, (size_t)NST * sizeof(double), hipHostMallocDefault));
+        c->h_stat_n = (size_t)NST;
+    }
+    const double *h = c->h_stat;
+    // device flags: d_info[0] the FP32 factorization's info, d_info[3] the
+    // rounding of A overflowed, d_info[4] the rounding of B or of a residual
+    int *a_ovf = c->d_info + 3, *r_ovf = c->d_info + 4;
     double anrm = 0.0, best = INFINITY;
-    int stall = 0, ovf = 0, h_ovf = 0;
+    int stall = 0, h_ovf = 0;
     auto fall_back = [&](int code) -> int {
         *iter = code;
         int r = mi_dpotrf_tiles(c, N, NB, dA, info);
@@ -908,29 +947,29 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
 
     // span on the host clock: the nested blocking calls use the context's events
     const auto t_start = std::chrono::steady_clock::now();
-    // A -> FP32 tiles (the lower triangle; the rest of the image is zeroed)
-    k_flag_clear<<<1, 1, 0, s>>>(c->d_info);
+    // A -> FP32 tiles (the lower triangle; the rest of the image is zeroed).
+    // Nothing below waits for the host before the first residual: the
+    // overflow of this rounding and the info of the FP32 factorization are
+    // read there, with the first norms, and decided on in the order
+    // dsposv does (rounding, then factorization).
+    k_flag_clear<<<1, 1, 0, s>>>(a_ovf);
     {
         // four runs of four elements per thread and tile part
         const int per = (int)std::max<int64_t>(1, ((int64_t)NB * NB + 4095) / 4096);
         const int64_t nblk = (int64_t)MT * (MT + 1) / 2 * per;
         if (nblk > 0x7fffffff) { set_error("dlag2s grid too large"); rc = -3; goto done; }
         MI_LAUNCH(c, s, "dlag2s", 0.0, 12.0 * (double)ntile_el,
-                  k_d2s_lower<<<(unsigned)nblk, 256, 0, s>>>(dA, L32, N, NB, MT, per, c->d_info));
+                  k_d2s_lower<<<(unsigned)nblk, 256, 0, s>>>(dA, L32, N, NB, MT, per, a_ovf));
     }
-    MX_CHECK(hipMemcpyAsync(&ovf, c->d_info, sizeof(int), hipMemcpyDeviceToHost, s), done);
-    MX_CHECK(hipStreamSynchronize(s), done);
-    if (ovf) { rc = fall_back(-2); goto done; }
     {
         int sinfo = 0;
-        rc = spotrf_tiles_split(c, N, NB, L32, &sinfo, prec == 1);
+        rc = spotrf_tiles_split(c, N, NB, L32, &sinfo, prec == 1, false);
         if (rc) goto done;
-        if (sinfo) { rc = fall_back(-3); goto done; }
     }
     {
-        auto solve32 = [&]() -> int {
+        auto solve32 = [&](bool reset_done) -> int {
             int r;
-            if ((r = trsm_enqueue<float>(c, s, 111, N, NB, L32, W32, NRHS, R32, N))) return r;
+            if ((r = trsm_enqueue<float>(c, s, 111, N, NB, L32, W32, NRHS, R32, N, reset_done))) return r;
             return trsm_enqueue<float>(c, s, 112, N, NB, L32, W32, NRHS, R32, N);
         };
         double cte = 0.0;                         // ||A||inf eps sqrt(N), from the first residual
@@ -940,9 +979,9 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
         if ((rc = trsm_prepare<float>(c, s, 0, N, NB, L32, W32))) goto done;
         // X0 = L32^-T L32^-1 B; the overflow flag covers the rounding of B
         // here and of every residual below (dlag2s: ITER = -2)
-        k_flag_clear<<<1, 1, 0, s>>>(c->d_info);
-        k_d2s_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(dB, ldb, R32, N, N, NRHS, c->d_info);
-        if ((rc = solve32())) goto done;
+        k_flag_clear<<<1, 1, 0, s>>>(r_ovf);
+        k_d2s_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(dB, ldb, R32, N, N, NRHS, r_ovf);
+        if ((rc = solve32(false))) goto done;
         k_s2d_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R32, N, dX, ldx, N, NRHS, 0);
         for (it = 0; it <= ITERMAX; it++) {
             // the first residual also yields the row sums of |A|, hence ||A||inf
@@ -951,11 +990,24 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
             colamax(s, R, N, N, NRHS, nrm);
             colamax(s, dX, ldx, N, NRHS, nrm + NRHS);
             if (it == 0) colamax(s, rs, N, N, 1, nrm + 2 * NRHS);
-            MX_CHECK(hipMemcpyAsync(h.data(), nrm, (2 * NRHS + 1) * sizeof(double), hipMemcpyDeviceToHost, s),
-                     done);
-            MX_CHECK(hipMemcpyAsync(&h_ovf, c->d_info, sizeof(int), hipMemcpyDeviceToHost, s), done);
+            k_mx_status<<<1, 1, 0, s>>>(nrm + 2 * NRHS + 1, c->d_info, a_ovf, r_ovf, trsm_abort_word(c));
+            MX_CHECK(hipMemcpyAsync(c->h_stat, nrm, (size_t)NST * sizeof(double), hipMemcpyDeviceToHost, s), done);
+            // the next correction's rounding and hand-off reset, queued
+            // before the wait (unused when this step ends the iteration; the
+            // status copied above does not include this rounding)
+            if (it < ITERMAX) {
+                k_d2s_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R, N, R32, N, N, NRHS, r_ovf);
+                if ((rc = trsm_reset<float>(c, s, N, NB))) goto done;
+            }
             MX_CHECK(hipStreamSynchronize(s), done);
-            if ((rc = trsm_status(c))) goto done;
+            if (it == 0) {
+                // decided in dsposv's order: the rounding of A, then the
+                // FP32 factorization (what followed them is discarded)
+                if (h[2 * NRHS + 1] != 0.0) { rc = fall_back(-2); goto done; }
+                if (h[2 * NRHS + 3] != 0.0) { rc = fall_back(-3); goto done; }
+            }
+            if (h[2 * NRHS + 4] != 0.0) { rc = trsm_abort_error(); goto done; }
+            h_ovf = h[2 * NRHS + 2] != 0.0;
             if (h_ovf) break;                     // B or a residual overflowed FP32
             if (it == 0) {
                 anrm = h[2 * NRHS];
@@ -974,8 +1026,7 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
             stall = (worst > 0.5 * best) ? stall + 1 : 0;
             best = std::min(best, worst);
             if (stall >= 2 || !(worst < INFINITY)) break;
-            k_d2s_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R, N, R32, N, N, NRHS, c->d_info);
-            if ((rc = solve32())) goto done;
+            if ((rc = solve32(true))) goto done;
             k_s2d_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R32, N, dX, ldx, N, NRHS, 1);
         }
         if (h_ovf) {

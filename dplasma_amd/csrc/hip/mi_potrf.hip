@@ -1770,7 +1770,7 @@ constexpr int half_tile_hmb(int NB) { return (NB >= 512 && NB % 64 == 0) ? NB / 
 
 template <class T>
 int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws, bool ring = false, int hmb = 0,
-                T *img_ext = nullptr, int rec_from = 0)
+                T *img_ext = nullptr, int rec_from = 0, bool wait = true)
 {
     *info = 0;
     if (N < 0 || NB <= 0) { set_error("bad sizes N=%d NB=%d", N, NB); return -2; }
@@ -1849,6 +1849,7 @@ int potrf_tiles(Ctx *c, int N, int NB, T *dT, int *info, bool bf16_split, T *ws,
     }
     MI_CHECK(hipStreamWaitEvent(c->s_main, pk, 0));
     MI_CHECK(hipEventRecord(c->t1, c->s_main));
+    if (!wait) return 0;                           // info stays in d_info[0]
     int h_info = 0;
     MI_CHECK(hipMemcpyAsync(&h_info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->s_main));
     if ((rc = mi_synchronize(static_cast<mi_ctx_t *>(c)))) return rc;
@@ -1922,11 +1923,11 @@ int task_update_fp64(Ctx *c, hipStream_t s, double *dT, int N, int NB, int m, in
 }
 size_t task_winv_elems(int NB) { return potrf_ws_elems(NB) / 2; }
 
-int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split)
+int spotrf_tiles_split(Ctx *c, int N, int NB, float *dT, int *info, bool bf16_split, bool wait)
 {
     const int MT = NB > 0 ? (N + NB - 1) / NB : 0;
     return potrf_tiles<float>(c, N, NB, dT, info, bf16_split, nullptr, false, bf16_split ? half_tile_hmb(NB) : 0,
-                              nullptr, MT / 2);
+                              nullptr, MT / 2, wait);
 }
 }  // namespace mi
 

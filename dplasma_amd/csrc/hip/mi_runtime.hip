@@ -197,6 +197,7 @@ void mi_ctx_destroy(mi_ctx_t *c)
     (void)hipEventDestroy(c->t1);
     (void)hipFree(c->d_info);
     (void)hipFree(c->d_scratch);
+    if (c->h_stat) (void)hipHostFree(c->h_stat);
     for (auto &b : c->ws)
         if (b.p) (void)hipFree(b.p);
     {

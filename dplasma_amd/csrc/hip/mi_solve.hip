@@ -1174,7 +1174,7 @@ int trsm_prepare(Ctx *c, hipStream_t s, int unit, int N, int NB, const T *dT, T 
 // B <- op(L)^{-1} B for all NRHS columns, enqueued on s (W from trsm_prepare).
 template <class T>
 int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, const T *W, int NRHS, T *dB,
-                 int ldb)
+                 int ldb, bool first_reset_done)
 {
     const int MT = (N + NB - 1) / NB;
     if (NB % 32 == 0 && g_wave_solve) {
@@ -1191,7 +1191,8 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
         for (int j0 = 0; j0 < NRHS; j0 += RHSB) {
             const int nr = (NRHS - j0 < RHSB) ? NRHS - j0 : RHSB;
             T *Bj = dB + (int64_t)j0 * ldb;
-            k_wave_reset<T><<<(unsigned)std::min<int64_t>((nst + 255) / 256, 1024), 256, 0, s>>>(xs, nst, ticket);
+            if (j0 > 0 || !first_reset_done)
+                k_wave_reset<T><<<(unsigned)std::min<int64_t>((nst + 255) / 256, 1024), 256, 0, s>>>(xs, nst, ticket);
             MI_LAUNCH(c, s, SN<T>::wave, 1.0 * N * N * nr, 0.5 * N * (double)N * sizeof(T),
                       (trans == 111
                        ? k_trsm_wave<T, false, 8><<<nwg, 512, 0, s>>>(dT, N, NB, MT, W128, Mf, Bj, ldb, nr, xs,
@@ -1212,6 +1213,23 @@ int trsm_enqueue(Ctx *c, hipStream_t s, int trans, int N, int NB, const T *dT, c
     return 0;
 }
 
+// The reset that trsm_enqueue does before its first chunk, enqueued ahead
+// (no-op when the solve does not use the wavefront kernels).
+template <class T>
+int trsm_reset(Ctx *c, hipStream_t s, int N, int NB)
+{
+    if (!(NB % 32 == 0 && g_wave_solve)) return 0;
+    const int nwg = (N + WV_RB - 1) / WV_RB;
+    int rc;
+    if ((rc = ws_uncached(c, c->wv_ticket, 2 * sizeof(int)))) return rc;
+    if ((rc = ws_uncached(c, c->wv_stage, (size_t)nwg * WV_RB * RHSB * sizeof(T)))) return rc;
+    const int64_t nst = (int64_t)nwg * WV_RB * RHSB;
+    k_wave_reset<T><<<(unsigned)std::min<int64_t>((nst + 255) / 256, 1024), 256, 0, s>>>(
+        (T *)c->wv_stage.p, nst, (int *)c->wv_ticket.p);
+    MI_CHECK(hipGetLastError());
+    return 0;
+}
+
 // Clears the abort word of the wavefront solve on s (before the first
 // trsm_enqueue of a blocking call).
 int trsm_begin(Ctx *c, hipStream_t s)
@@ -1225,24 +1243,30 @@ int trsm_begin(Ctx *c, hipStream_t s)
 
 // After the context's streams are synchronized: -5 when a wait of the
 // wavefront solve gave up (its results are not valid), else 0.
+int trsm_abort_error()
+{
+    set_error("triangular solve: a hand-off wait between workgroups exceeded its time limit");
+    return -5;
+}
+
 int trsm_status(Ctx *c)
 {
     int ab = 0;
     if (!c->wv_ticket.p) return 0;
     MI_CHECK(hipMemcpy(&ab, (int *)c->wv_ticket.p + 1, sizeof(int), hipMemcpyDeviceToHost));
-    if (ab) {
-        set_error("triangular solve: a hand-off wait between workgroups exceeded its time limit");
-        return -5;
-    }
-    return 0;
+    return ab ? trsm_abort_error() : 0;
 }
+
+const int *trsm_abort_word(Ctx *c) { return c->wv_ticket.p ? (const int *)c->wv_ticket.p + 1 : nullptr; }
 
 template int trsm_prepare<double>(Ctx *, hipStream_t, int, int, int, const double *, double *);
 template int trsm_prepare<float>(Ctx *, hipStream_t, int, int, int, const float *, float *);
 template int trsm_enqueue<double>(Ctx *, hipStream_t, int, int, int, const double *, const double *, int,
-                                  double *, int);
+                                  double *, int, bool);
 template int trsm_enqueue<float>(Ctx *, hipStream_t, int, int, int, const float *, const float *, int, float *,
-                                 int);
+                                 int, bool);
+template int trsm_reset<double>(Ctx *, hipStream_t, int, int);
+template int trsm_reset<float>(Ctx *, hipStream_t, int, int);
 
 }  // namespace mi
 

@@ -8,7 +8,8 @@
     thread, and run at the same time from two threads, give the answers each
     gives alone;
   * the wavefront solve's bounded hand-off wait, forced to give up at once
-    from the host: the call returns -5 and the next call is correct again.
+    from the host (dpotrs and the mixed solve): the call returns -5 and the
+    next call is correct again.
 Every kernel in these tests exits normally."""
 import ctypes
 import threading
@@ -186,6 +187,33 @@ def test_solve_wait_limit_gives_up_and_recovers(dev, po, dp):
     assert mi.mi_set_solve_wait_limit(-1.0) == -2
     dev.free(dT)
     dev.free(dB)
+
+
+def test_dsposv_wait_limit_gives_up_and_recovers(dev, po, dp):
+    """The mixed solve reads the abort word with its first norms: a zero wait
+    limit makes its FP32 solves give up, the call returns -5, and the next
+    call (limit restored) converges."""
+    mi, _ = dp.libs()
+    N, NB, nrhs = 1024, 256, 20          # two chunks of right-hand sides
+    A = spd(N, 7)
+    T = po.full_to_symtiles(A, NB)
+    B = np.random.default_rng(4).uniform(-0.5, 0.5, (N, nrhs))
+    dT, dB = dev.upload(T), dev.upload(colmajor(B))
+    dX = dev.alloc(N * nrhs * 8)
+    it, info = ctypes.c_int(-99), ctypes.c_int(-99)
+    assert mi.mi_set_solve_wait_limit(0.0) == 0
+    try:
+        rc = dev.M.mi_dsposv_tiles(dev.h, N, NB, dT, nrhs, dB, N, dX, N, 1, ctypes.byref(it), ctypes.byref(info))
+        msg = dp.last_error()
+    finally:
+        assert mi.mi_set_solve_wait_limit(10.0) == 0
+    assert rc == -5, (rc, msg)
+    assert "time limit" in msg
+    for p in (dT, dB, dX):
+        dev.free(p)
+    X, it2, info2 = run_dsposv(dev, T, N, NB, B, 1)
+    assert info2 == 0 and 0 <= it2 <= 30, it2
+    assert np.abs(X - np.linalg.solve(A, B)).max() < 1e-10
 
 
 def test_dpotrf_image_larger_than_hbm_returns_negative(ctx, dp):
