@@ -703,54 +703,58 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
         }
     }
     const int frc = min(fr, max(nr, 1) - 1);
-    V4 acc[TPW];
+    // Catch-up layout: wave (rg, kh) = (wave / 2, wave % 2) takes rows
+    // [32 rg, 32 rg + 32) of the block (two MFMA row tiles, rows
+    // wv_row<TRANS, 2>) against one half of each earlier block's columns
+    // (slices hb = 2 kh, 2 kh + 1): two rows per lane give 16-byte loads of L
+    // forward, and each wave reads only its half of X.  The two halves of y
+    // meet in LDS before z = W_g y, which (like the link product and the
+    // stores) uses the layout of RW rows per wave.
+    static_assert(NW == 8, "the catch-up layout pairs eight waves");
+    const int rg = wave >> 1, kh = wave & 1;
+    const int rc = r0 + 32 * rg;                 // catch-up rows of this wave
+    V4 acc[2];
 #pragma unroll
-    for (int t = 0; t < TPW; t++)
+    for (int t = 0; t < 2; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int row = rw + wv_row<TRANS, TPW>(t, MF<T>::row(q, fk));
+            const int row = rc + wv_row<TRANS, 2>(t, MF<T>::row(q, fk));
             const T v = B[(int64_t)frc * ldb + min(row, N - 1)];
-            acc[t][q] = (row < N && fr < nr) ? v : T(0);
+            acc[t][q] = (kh == 0 && row < N && fr < nr) ? v : T(0);
         }
 
-    // a-operands of one other block h: op(L)(my row wv_row(t, fr), column
-    // 32 hb + wv_k(kq, fk) of the block)
-    T la[4][8][TPW];
+    // a-operands of one other block h: op(L)(row rc + wv_row(t, fr), column
+    // 32 (2 kh + hl) + wv_k(kq, fk) of the block) as la[hl][kq][t]
+    T la[2][8][2];
     // (the addresses are clamped into the matrix).  FP64 zeroes what lies
     // outside the matrix as the values arrive; FP32, whose loads run two
     // blocks ahead, leaves that to mask_a, once the values are needed.
-    constexpr bool MASK_AT_LOAD = sizeof(T) == 8;
-    auto load_a = [&](T (&la)[4][8][TPW], int h) {
+    constexpr bool MASK_AT_LOAD = false;
+    auto load_a = [&](T (&la)[2][8][2], int h) {
 #pragma unroll
-        for (int hb = 0; hb < 4; hb++) {
-            const int c0 = h * WV_RB + 32 * hb;
+        for (int hl = 0; hl < 2; hl++) {
+            const int c0 = h * WV_RB + 32 * (2 * kh + hl);
             const int cc = min(c0, N - 1) & ~31;     // a valid 32-aligned block when c0 >= N
-            const int rr = min(rw, N - 1) & ~(RW - 1);
+            const int rr = min(rc, N - 1) & ~31;
             auto put = [&](int kq, int t, T v) {
                 if (MASK_AT_LOAD) {
-                    const int kk = wv_k<TRANS>(kq, fk), row = rw + wv_row<TRANS, TPW>(t, fr);
+                    const int kk = wv_k<TRANS>(kq, fk), row = rc + wv_row<TRANS, 2>(t, fr);
                     v = (c0 < N && c0 + kk < N && row < N) ? v : T(0);
                 }
-                la[hb][kq][t] = v;
+                la[hl][kq][t] = v;
             };
             if (!TRANS) {
-                if constexpr (TPW == 2) {
-                    const T *pA = sym_elem(Tt, MT, NB, rr, cc) + 2 * fr;
+                const T *pA = sym_elem(Tt, MT, NB, rr, cc) + 2 * fr;
 #pragma unroll
-                    for (int kq = 0; kq < 8; kq++) {
-                        const V2 v = *(const V2 *)(pA + (int64_t)(4 * kq + fk) * NB);
-                        put(kq, 0, v.x);
-                        put(kq, 1, v.y);
-                    }
-                } else {
-                    const T *pA = sym_elem(Tt, MT, NB, rr, cc) + fr;
-#pragma unroll
-                    for (int kq = 0; kq < 8; kq++) put(kq, 0, pA[(int64_t)(4 * kq + fk) * NB]);
+                for (int kq = 0; kq < 8; kq++) {
+                    const V2 v = *(const V2 *)(pA + (int64_t)(4 * kq + fk) * NB);
+                    put(kq, 0, v.x);
+                    put(kq, 1, v.y);
                 }
             } else {
                 const T *pA = sym_elem(Tt, MT, NB, cc, rr) + 4 * fk;
 #pragma unroll
-                for (int t = 0; t < TPW; t++)
+                for (int t = 0; t < 2; t++)
 #pragma unroll
                     for (int kg = 0; kg < 2; kg++) {
                         T v[4];
@@ -761,22 +765,23 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
             }
         }
     };
-    auto mask_a = [&](T (&la)[4][8][TPW], int h) {
-        if ((h + 1) * WV_RB <= N && rw + RW <= N) return;     // wave-uniform
+    auto mask_a = [&](T (&la)[2][8][2], int h) {
+        if ((h + 1) * WV_RB <= N && rc + 32 <= N) return;     // wave-uniform
 #pragma unroll
-        for (int hb = 0; hb < 4; hb++) {
-            const int c0 = h * WV_RB + 32 * hb;
+        for (int hl = 0; hl < 2; hl++) {
+            const int c0 = h * WV_RB + 32 * (2 * kh + hl);
 #pragma unroll
             for (int kq = 0; kq < 8; kq++)
 #pragma unroll
-                for (int t = 0; t < TPW; t++) {
-                    const int kk = wv_k<TRANS>(kq, fk), row = rw + wv_row<TRANS, TPW>(t, fr);
-                    if (!(c0 < N && c0 + kk < N && row < N)) la[hb][kq][t] = T(0);
+                for (int t = 0; t < 2; t++) {
+                    const int kk = wv_k<TRANS>(kq, fk), row = rc + wv_row<TRANS, 2>(t, fr);
+                    if (!(c0 < N && c0 + kk < N && row < N)) la[hl][kq][t] = T(0);
                 }
         }
     };
     // a-operands of the link matrix: M_g(RW w + wv_row(t, fr), 32 hb + wv_k(kq, fk)),
     // column-major forward, row-major backward
+    T lm[4][8][TPW];
     auto load_m = [&]() {
         const T *Mg = Mlink + (int64_t)g * 16384;
 #pragma unroll
@@ -786,10 +791,10 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
                 for (int kq = 0; kq < 8; kq++) {
                     if constexpr (TPW == 2) {
                         const V2 v = *(const V2 *)(Mg + (32 * hb + 4 * kq + fk) * 128 + RW * wave + 2 * fr);
-                        la[hb][kq][0] = v.x;
-                        la[hb][kq][1] = v.y;
+                        lm[hb][kq][0] = v.x;
+                        lm[hb][kq][1] = v.y;
                     } else {
-                        la[hb][kq][0] = Mg[(32 * hb + 4 * kq + fk) * 128 + RW * wave + fr];
+                        lm[hb][kq][0] = Mg[(32 * hb + 4 * kq + fk) * 128 + RW * wave + fr];
                     }
                 }
             } else {
@@ -800,15 +805,15 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
                         T v[4];
                         wv_load4(Mg + (RW * wave + 16 * t + fr) * 128 + 32 * hb + 16 * kg + 4 * fk, v);
 #pragma unroll
-                        for (int e = 0; e < 4; e++) la[hb][4 * kg + e][t] = v[e];
+                        for (int e = 0; e < 4; e++) lm[hb][4 * kg + e][t] = v[e];
                     }
             }
         }
     };
-    // -X(block h) as b-operands, once every value has been stored: for the
-    // last block (fin) lanes 0..NW-1 first poll one value of each producer
-    // wave; then the wave loads its values, and loads again while any of
-    // them is still the sentinel
+    // -X(block h), slices [hb0, hb0 + NH), as b-operands, once every value
+    // has been stored: for the last block (fin) lanes 0..NW-1 first poll one
+    // value of each producer wave; then the wave loads its values, and
+    // loads again while any of them is still the sentinel
     // true once this wait has to end without its values: the abort word is
     // set, or this wait outlived its budget (then it sets the word)
     int *abort_word = ticket + 1;
@@ -825,7 +830,8 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
         __hip_atomic_store(wp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return true;
     };
-    auto wait_x = [&](int h, bool fin, T (&bx)[4][8]) {
+    auto wait_x = [&](int h, bool fin, int hb0, auto &bx) {
+        constexpr int NH = sizeof(bx) / sizeof(bx[0]);
         const T *xh = xs + (int64_t)h * (WV_RB * RHSB);
         const long long t_end = (long long)__builtin_amdgcn_s_memrealtime() + wait_ticks;
         if (wait_ticks == 0) (void)give_up(t_end);   // a zero budget ends every wait at once
@@ -838,26 +844,26 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
             int ok = 1;
             if constexpr (sizeof(T) == 8) {
 #pragma unroll
-                for (int hb = 0; hb < 4; hb++)
+                for (int hl = 0; hl < NH; hl++)
 #pragma unroll
                     for (int kq = 0; kq < 8; kq++) {
-                        const U b = wv_load(xh + (32 * hb + wv_k<TRANS>(kq, fk)) * RHSB + fr);
+                        const U b = wv_load(xh + (32 * (hb0 + hl) + wv_k<TRANS>(kq, fk)) * RHSB + fr);
                         ok &= (b != WvSent<T>::v);
-                        bx[hb][kq] = __builtin_bit_cast(T, b);
+                        bx[hl][kq] = __builtin_bit_cast(T, b);
                     }
             } else {
 #pragma unroll
-                for (int hb = 0; hb < 4; hb++)
+                for (int hl = 0; hl < NH; hl++)
 #pragma unroll
                     for (int kp = 0; kp < 4; kp++) {
                         const unsigned long long *p =
-                            (const unsigned long long *)(xh + ((hb * 4 + kp) * 64 + lane) * 2);
+                            (const unsigned long long *)(xh + (((hb0 + hl) * 4 + kp) * 64 + lane) * 2);
                         asm volatile("" : "+v"(p));
                         const unsigned long long b = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
                         ok &= (lo != (unsigned)WvSent<float>::v) & (hi != (unsigned)WvSent<float>::v);
-                        bx[hb][2 * kp] = __builtin_bit_cast(T, lo);
-                        bx[hb][2 * kp + 1] = __builtin_bit_cast(T, hi);
+                        bx[hl][2 * kp] = __builtin_bit_cast(T, lo);
+                        bx[hl][2 * kp + 1] = __builtin_bit_cast(T, hi);
                     }
             }
             if (__all(ok)) break;
@@ -865,22 +871,22 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
             __builtin_amdgcn_s_sleep(1);
         }
 #pragma unroll
-        for (int hb = 0; hb < 4; hb++)
+        for (int hl = 0; hl < NH; hl++)
 #pragma unroll
             for (int kq = 0; kq < 8; kq++) {
-                const int r = h * WV_RB + 32 * hb + wv_k<TRANS>(kq, fk);
-                bx[hb][kq] = (fr < nr && r < N) ? -bx[hb][kq] : T(0);
+                const int r = h * WV_RB + 32 * (hb0 + hl) + wv_k<TRANS>(kq, fk);
+                bx[hl][kq] = (fr < nr && r < N) ? -bx[hl][kq] : T(0);
             }
     };
     const int nprev = TRANS ? nwg - 1 - g : g;
     auto blk = [&](int s) { return TRANS ? nwg - 1 - s : s; };   // the s-th earlier block
-    auto mfma_block = [&](const T (&l)[4][8][TPW], const T (&bx)[4][8]) {
+    auto mfma_block = [&](const T (&l)[2][8][2], const T (&bx)[2][8]) {
 #pragma unroll
-        for (int hb = 0; hb < 4; hb++)
+        for (int hl = 0; hl < 2; hl++)
 #pragma unroll
             for (int kq = 0; kq < 8; kq++)
 #pragma unroll
-                for (int t = 0; t < TPW; t++) acc[t] = MF<T>::mfma(l[hb][kq][t], bx[hb][kq], acc[t]);
+                for (int t = 0; t < 2; t++) acc[t] = MF<T>::mfma(l[hl][kq][t], bx[hl][kq], acc[t]);
     };
     if constexpr (sizeof(T) == 4) {
         // Every earlier block but the last is long finished by the time it
@@ -892,31 +898,31 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
         // unconditional (the last block is read once more at the end) so
         // that no register holding a load in flight is copied.  A read-ahead
         // X that still holds a sentinel is read again by the bounded wait.
-        typedef unsigned long long X2[4][4];
+        typedef unsigned long long X2[2][4];
         typedef __attribute__((address_space(1))) const unsigned long long GU;
-        T lb[4][8][TPW];
+        T lb[2][8][2];
         X2 xa, xb;
         auto read_x = [&](int h, X2 &xr) {
             const T *xh = xs + (int64_t)h * (WV_RB * RHSB);
 #pragma unroll
-            for (int hb = 0; hb < 4; hb++)
+            for (int hl = 0; hl < 2; hl++)
 #pragma unroll
                 for (int kp = 0; kp < 4; kp++) {
-                    GU *p = (GU *)(xh + ((hb * 4 + kp) * 64 + lane) * 2);
-                    xr[hb][kp] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    GU *p = (GU *)(xh + (((2 * kh + hl) * 4 + kp) * 64 + lane) * 2);
+                    xr[hl][kp] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
         };
         auto ready = [&](const X2 &xr) {
             int ok = 1;
 #pragma unroll
-            for (int hb = 0; hb < 4; hb++)
+            for (int hl = 0; hl < 2; hl++)
 #pragma unroll
                 for (int kp = 0; kp < 4; kp++)
-                    ok &= ((unsigned)xr[hb][kp] != (unsigned)WvSent<float>::v) &
-                          ((unsigned)(xr[hb][kp] >> 32) != (unsigned)WvSent<float>::v);
+                    ok &= ((unsigned)xr[hl][kp] != (unsigned)WvSent<float>::v) &
+                          ((unsigned)(xr[hl][kp] >> 32) != (unsigned)WvSent<float>::v);
             return __all(ok);
         };
-        auto consume = [&](int s, T (&lc)[4][8][TPW], X2 &xc) {
+        auto consume = [&](int s, T (&lc)[2][8][2], X2 &xc) {
             if (!ready(xc)) {
                 // the bounded wait of wait_x, reading into the same registers
                 const long long t_end = (long long)__builtin_amdgcn_s_memrealtime() + wait_ticks;
@@ -928,25 +934,25 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
                     if (ready(xc)) break;
                 }
             }
-            T bx[4][8];
+            T bx[2][8];
 #pragma unroll
-            for (int hb = 0; hb < 4; hb++)
+            for (int hl = 0; hl < 2; hl++)
 #pragma unroll
                 for (int kp = 0; kp < 4; kp++) {
-                    bx[hb][2 * kp] = __builtin_bit_cast(T, (unsigned)xc[hb][kp]);
-                    bx[hb][2 * kp + 1] = __builtin_bit_cast(T, (unsigned)(xc[hb][kp] >> 32));
+                    bx[hl][2 * kp] = __builtin_bit_cast(T, (unsigned)xc[hl][kp]);
+                    bx[hl][2 * kp + 1] = __builtin_bit_cast(T, (unsigned)(xc[hl][kp] >> 32));
                 }
 #pragma unroll
-            for (int hb = 0; hb < 4; hb++)
+            for (int hl = 0; hl < 2; hl++)
 #pragma unroll
                 for (int kq = 0; kq < 8; kq++) {
-                    const int r = blk(s) * WV_RB + 32 * hb + wv_k<TRANS>(kq, fk);
-                    bx[hb][kq] = (fr < nr && r < N) ? -bx[hb][kq] : T(0);
+                    const int r = blk(s) * WV_RB + 32 * (2 * kh + hl) + wv_k<TRANS>(kq, fk);
+                    bx[hl][kq] = (fr < nr && r < N) ? -bx[hl][kq] : T(0);
                 }
             mask_a(lc, blk(s));
             mfma_block(lc, bx);
         };
-        if (nprev > 1 && rw < N) {
+        if (nprev > 1 && rc < N) {
             const int last = nprev - 2;           // the last block of this loop
             load_a(la, blk(0));
             read_x(blk(0), xa);
@@ -962,11 +968,12 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
             }
         }
     } else {
-        if (nprev > 1 && rw < N) load_a(la, blk(0));
+        if (nprev > 1 && rc < N) load_a(la, blk(0));
         for (int s = 0; s + 1 < nprev; s++) {
-            if (rw < N) {
-                T bx[4][8];
-                wait_x(blk(s), false, bx);
+            if (rc < N) {
+                T bx[2][8];
+                wait_x(blk(s), false, 2 * kh, bx);
+                mask_a(la, blk(s));
                 mfma_block(la, bx);
                 if (s + 2 < nprev) load_a(la, blk(s + 1));
             }
@@ -974,11 +981,22 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
     }
     if (nprev > 0 && rw < N) load_m();
 
-    // z = op(W_g) y
+    // y = the two column halves summed (the second half added to the first
+    // in LDS, one order for every run), then z = op(W_g) y
+    if (kh == 1)
 #pragma unroll
-    for (int t = 0; t < TPW; t++)
+        for (int t = 0; t < 2; t++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) Ys[RW * wave + wv_row<TRANS, TPW>(t, MF<T>::row(q, fk))][fr] = acc[t][q];
+            for (int q = 0; q < 4; q++) Ys[32 * rg + wv_row<TRANS, 2>(t, MF<T>::row(q, fk))][fr] = acc[t][q];
+    __syncthreads();
+    if (kh == 0)
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                T &y = Ys[32 * rg + wv_row<TRANS, 2>(t, MF<T>::row(q, fk))][fr];
+                y = acc[t][q] + y;
+            }
     __syncthreads();
     V4 x[TPW];
 #pragma unroll
@@ -994,13 +1012,13 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
     // x_g = z - M_g X(last block)
     if (nprev > 0 && rw < N) {
         T bx[4][8];
-        wait_x(TRANS ? g + 1 : g - 1, true, bx);
+        wait_x(TRANS ? g + 1 : g - 1, true, 0, bx);
 #pragma unroll
         for (int hb = 0; hb < 4; hb++)
 #pragma unroll
             for (int kq = 0; kq < 8; kq++)
 #pragma unroll
-                for (int t = 0; t < TPW; t++) x[t] = MF<T>::mfma(la[hb][kq][t], bx[hb][kq], x[t]);
+                for (int t = 0; t < TPW; t++) x[t] = MF<T>::mfma(lm[hb][kq][t], bx[hb][kq], x[t]);
     }
     T *xg = xs + (int64_t)g * (WV_RB * RHSB);
 #pragma unroll
