@@ -765,6 +765,18 @@ inline void colamax(hipStream_t s, const double *V, int ld, int M, int nc, doubl
 
 __global__ void k_flag_clear(int *p) { *p = 0; }
 
+// Waits for s by polling: the host decides the next refinement step as soon
+// as the norms are there (a blocking wait that has slept through the
+// factorization wakes up ~0.1 ms late).  Every kernel queued here ends by
+// itself (the solve's hand-off waits are bounded), so the poll ends.
+static hipError_t spin_sync(hipStream_t s)
+{
+    hipError_t e;
+    while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
+    }
+    return e;
+}
+
 // Status words of one refinement step, as doubles next to the norms (one
 // copy to the host): the rounding of A overflowed, the rounding of B or of a
 // residual overflowed, the FP32 factorization's info, a hand-off wait of the
@@ -999,7 +1011,7 @@ int mi_dsposv_tiles(mi_ctx_t *c, int N, int NB, double *dA, int NRHS, const doub
                 k_d2s_cm<<<grid_of((int64_t)N * NRHS), 256, 0, s>>>(R, N, R32, N, N, NRHS, r_ovf);
                 if ((rc = trsm_reset<float>(c, s, N, NB))) goto done;
             }
-            MX_CHECK(hipStreamSynchronize(s), done);
+            MX_CHECK(spin_sync(s), done);
             if (it == 0) {
                 // decided in dsposv's order: the rounding of A, then the
                 // FP32 factorization (what followed them is discarded)
