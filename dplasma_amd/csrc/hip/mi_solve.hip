@@ -811,9 +811,10 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
         }
     };
     // -X(block h), slices [hb0, hb0 + NH), as b-operands, once every value
-    // has been stored: for the last block (fin) lanes 0..NW-1 first poll one
-    // value of each producer wave; then the wave loads its values, and
-    // loads again while any of them is still the sentinel
+    // has been stored: the wave loads its values, and loads them again while
+    // any of them is still the sentinel (for the last block too: a cheaper
+    // poll of one value per producer wave first costs one more round trip
+    // per link, 1-2 % of a sweep)
     // true once this wait has to end without its values: the abort word is
     // set, or this wait outlived its budget (then it sets the word)
     int *abort_word = ticket + 1;
@@ -830,16 +831,11 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
         __hip_atomic_store(wp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return true;
     };
-    auto wait_x = [&](int h, bool fin, int hb0, auto &bx) {
+    auto wait_x = [&](int h, int hb0, auto &bx) {
         constexpr int NH = sizeof(bx) / sizeof(bx[0]);
         const T *xh = xs + (int64_t)h * (WV_RB * RHSB);
         const long long t_end = (long long)__builtin_amdgcn_s_memrealtime() + wait_ticks;
         if (wait_ticks == 0) (void)give_up(t_end);   // a zero budget ends every wait at once
-        if (fin && lane < NW)
-            while (wv_load(xh + wv_xs<T, TRANS>(RW * lane + RW - 1, RHSB - 1)) == WvSent<T>::v) {
-                if (give_up(t_end)) break;
-                __builtin_amdgcn_s_sleep(1);
-            }
         for (;;) {
             int ok = 1;
             if constexpr (sizeof(T) == 8) {
@@ -972,7 +968,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
         for (int s = 0; s + 1 < nprev; s++) {
             if (rc < N) {
                 T bx[2][8];
-                wait_x(blk(s), false, 2 * kh, bx);
+                wait_x(blk(s), 2 * kh, bx);
                 mask_a(la, blk(s));
                 mfma_block(la, bx);
                 if (s + 2 < nprev) load_a(la, blk(s + 1));
@@ -1012,7 +1008,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
     // x_g = z - M_g X(last block)
     if (nprev > 0 && rw < N) {
         T bx[4][8];
-        wait_x(TRANS ? g + 1 : g - 1, true, 0, bx);
+        wait_x(TRANS ? g + 1 : g - 1, 0, bx);
 #pragma unroll
         for (int hb = 0; hb < 4; hb++)
 #pragma unroll
