@@ -887,11 +887,13 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
     if constexpr (sizeof(T) == 4) {
         // Every earlier block but the last is long finished by the time it
         // is needed, so its X is read ahead like its L: the L operands and
-        // one read of X of block s + 1 are issued before block s is waited
-        // for and multiplied, two blocks in flight.  They are issued after
-        // the loads of block s and vector-memory loads complete in order, so
-        // waiting for block s leaves them outstanding.  The loads are
-        // unconditional (the last block is read once more at the end) so
+        // one read of X of blocks s + 1 and s + 2 are issued before block s
+        // is waited for and multiplied, three blocks in flight (the counter
+        // of outstanding loads, at most 63, holds about two and a half).
+        // They are issued after the loads of block s and vector-memory loads
+        // complete in order, so waiting for block s leaves them outstanding.
+        // The loads are unconditional (the last block is read again at the
+        // end) so
         // that no register holding a load in flight is copied.  A read-ahead
         // X that still holds a sentinel is read again by the bounded wait.
         typedef unsigned long long X2[2][4];
@@ -950,16 +952,24 @@ __global__ __launch_bounds__(64 * NW, 1) void k_trsm_wave(const T *__restrict__ 
         };
         if (nprev > 1 && rc < N) {
             const int last = nprev - 2;           // the last block of this loop
+            T lc[2][8][2];
+            X2 xc;
             load_a(la, blk(0));
             read_x(blk(0), xa);
+            load_a(lb, blk(min(1, last)));
+            read_x(blk(min(1, last)), xb);
             for (int s = 0;;) {
-                load_a(lb, blk(min(s + 1, last)));
-                read_x(blk(min(s + 1, last)), xb);
+                load_a(lc, blk(min(s + 2, last)));
+                read_x(blk(min(s + 2, last)), xc);
                 consume(s, la, xa);
                 if (++s > last) break;
-                load_a(la, blk(min(s + 1, last)));
-                read_x(blk(min(s + 1, last)), xa);
+                load_a(la, blk(min(s + 2, last)));
+                read_x(blk(min(s + 2, last)), xa);
                 consume(s, lb, xb);
+                if (++s > last) break;
+                load_a(lb, blk(min(s + 2, last)));
+                read_x(blk(min(s + 2, last)), xb);
+                consume(s, lc, xc);
                 if (++s > last) break;
             }
         }
